@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""GPP sigma throughput on one MI355X.
+
+One step is one full evaluation of the GPP sigma band loop (gppComplex.cpp
+:335-383) -- achtemp/asxtemp/acht_n1_loc/achstemp -- over the benchmark
+problem resident in HBM: number_bands=512, nvband=2, ncouls=32768,
+ngpown=1638, nfreq=3, seeded complex128 inputs (generated on the device).
+The metric counts one update per (n1, igp, ig, iw) tuple.
+
+Prints ONE JSON line.  `value` is the updates/s of the headline form; the
+other forms' numbers, their errors, per-stage GPU times (HIP events on the
+library's stream), the roofline of the dominant kernel and the reference's
+OpenMP CPU path timed on this host are attached.
+
+    python bench.py [--steps K] [--warmup W] [--form direct|...] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "doe-csfg-hackathon_amd")
+ORACLE = os.path.join(REPO, "oracle")
+sys.path.insert(0, PKG)
+
+METRIC = "GPP sigma updates/s (bands\u00b7ngpown\u00b7ncouls\u00b7nfreq), direct vs factored @\u22641e-10 rel err"
+NB, NV, NCOULS, NGPOWN, NFREQ, SEED = 512, 2, 32768, 1638, 3, 1
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix, dense
+PEAK_HBM_GBS = 8000.0
+FLOP_UNOCC = 33           # per unoccupied update, counted on the reference's expression
+FLOP_OCC = 62             # per occupied update (branch-1 path)
+
+
+def cpu_cores():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return min(16, os.cpu_count() or 1)
+
+
+def cpu_baseline():
+    """The reference's OpenMP build (gppOpenMP3.cpp, compiled from source into
+    oracle/_ref) on a bounded band sample of the benchmark shape; falls back
+    to the oracle's race-free OpenMP restatement when the reference binary is
+    absent.  Runs before the GPU is initialised."""
+    cores = cpu_cores()
+    nb_s = 32
+    ref_exe = os.path.join(ORACLE, "_ref", "gppOpenMP3.ex")
+    updates = nb_s * NGPOWN * NCOULS * NFREQ
+    if os.path.exists(ref_exe):
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        r = subprocess.run([ref_exe, str(nb_s), str(NV), str(NCOULS), "20"], capture_output=True, text=True,
+                           env=env, timeout=600)
+        m = re.search(r"Time Taken \*+= ([0-9.eE+-]+)", r.stdout)
+        if r.returncode == 0 and m:
+            secs = float(m.group(1))
+            return {"value": updates / secs, "unit": "updates/s", "cores": cores, "kind": "reference",
+                    "sample": f"reference gppOpenMP3 (g++ -O3 -fopenmp, {cores} threads) on '{nb_s} {NV} {NCOULS} 20': "
+                              f"{nb_s} of 512 bands, ngpown={NGPOWN}, nfreq=3, its deterministic init; "
+                              f"{updates:.3e} updates in {secs:.3f} s (its own loop timer)",
+                    "seconds": secs}
+    sys.path.insert(0, ORACLE)
+    import gpp_oracle
+    import pygpp
+    p = pygpp.Problem(nb_s, NV, NCOULS, NGPOWN, NFREQ)
+    arrs = pygpp.fill_seeded(p, SEED)
+    t0 = time.perf_counter()
+    gpp_oracle.sigma(arrs, nvband=NV, threads=cores)
+    secs = time.perf_counter() - t0
+    return {"value": updates / secs, "unit": "updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle restatement of gppComplex.cpp (race-free OpenMP, {cores} threads) on {nb_s} of 512 "
+                      f"bands at ncouls={NCOULS}, ngpown={NGPOWN}, nfreq=3, seeded; {updates:.3e} updates in {secs:.3f} s",
+            "seconds": secs}
+
+
+def load_traffic(kernel):
+    path = os.path.join(REPO, "profiles", f"traffic_{kernel}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def rel_err(a, b):
+    import numpy as np
+    a = np.atleast_1d(np.asarray(a, np.complex128))
+    b = np.atleast_1d(np.asarray(b, np.complex128))
+    w = 0.0
+    for x, y in zip(a, b):
+        for u, v in ((x.real, y.real), (x.imag, y.imag)):
+            w = max(w, abs(u - v) / max(abs(v), 1e-6 * abs(y), 1e-300))
+    return w
+
+
+def time_form(ctx, form, steps, warmup):
+    for _ in range(warmup):
+        ctx.sigma(form)
+    stage_sum = {}
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = ctx.sigma(form)  # synchronous: returns after the results are on the host
+        for k, v in ctx.last_timing().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    return out, ms, {k: v / steps for k, v in stage_sum.items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--form", default="direct", choices=["direct", "direct-strict", "factored-f64", "factored-ozaki"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
+              "multi-GPU sharding of the G' columns is a design note, not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    import pygpp
+    prob = pygpp.Problem(NB, NV, NCOULS, NGPOWN, NFREQ)
+    updates = prob.updates
+    ctx = pygpp.Context(0)
+    ctx.generate_seeded(prob, SEED)
+
+    forms = {}
+    results = {}
+    for form in ["direct", "direct-strict"] if args.form.startswith("direct") else [args.form, "direct"]:
+        out, ms, stages = time_form(ctx, form, args.steps if form == args.form else max(1, args.steps // 2),
+                                    args.warmup)
+        results[form] = out
+        forms[form] = {"ms_per_step": ms, "updates_per_s": updates / (ms * 1e-3), "gpu_stage_ms": stages}
+    head = forms[args.form]
+    # agreement between forms (the direct forms are checked against the CPU oracle by the tests)
+    base = results["direct-strict"] if "direct-strict" in results else results["direct"]
+    for form, out in results.items():
+        forms[form]["max_rel_err_vs_" + ("direct-strict" if "direct-strict" in results else "direct")] = max(
+            rel_err(out[k], base[k]) for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp"))
+
+    # roofline of the dominant kernel of the headline form
+    if args.form.startswith("direct"):
+        kname = "direct_unoccupied_strict" if args.form == "direct-strict" else "direct_unoccupied"
+        kms = head["gpu_stage_ms"][kname]
+        flop = (NB - NV) * NGPOWN * NCOULS * NFREQ * FLOP_UNOCC
+        achieved = flop / (kms * 1e-3) / 1e12
+        tr = load_traffic("k_unocc")
+        roofline = {"bound": "mfma", "pipe": "FP64 VALU (FP64 vector peak = FP64 matrix peak on MI355X)",
+                    "kernel": kname, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / PEAK_FP64_TFLOPS,
+                    "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+                    "algorithmic_flop_per_launch": flop, "avg_launch_ms": kms}
+    else:
+        roofline = None
+
+    line = {
+        "metric": METRIC,
+        "value": head["updates_per_s"],
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic: seeded complex128 inputs (seed {SEED}, counter-based SplitMix64) generated in HBM",
+        "config": {"workload": f"GPP sigma {args.form}: number_bands={NB} nvband={NV} ncouls={NCOULS} "
+                               f"ngpown={NGPOWN} nfreq={NFREQ} (wx=3,4,5)",
+                   "form": args.form, "number_bands": NB, "nvband": NV, "ncouls": NCOULS, "ngpown": NGPOWN,
+                   "nfreq": NFREQ, "updates_per_step": updates},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = head["updates_per_s"] / cpu["value"]
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

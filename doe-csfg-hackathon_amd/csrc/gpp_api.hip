@@ -1,0 +1,616 @@
+// gpp_api.hip -- the extern "C" boundary of libgpp_hip.so (include/gpp.h).
+//
+// Host-side only: argument checking, device buffers, HIP streams/events and
+// dispatch to the kernel launchers of gpp_direct.hip / gpp_factored.hip.
+// Nothing here prints, exits or falls back to a CPU path: every failure is a
+// gpp_status plus a thread-local message.
+#include <mutex>
+#include <new>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "../../include/gpp.h"
+#include "gpp_internal.hpp"
+#include "gpp_rng.hpp"
+
+using gpp::cplx;
+
+static_assert(sizeof(gpp_complex) == sizeof(cplx), "gpp_complex must be 16 bytes");
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+gpp_status fail(gpp_status st, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+gpp_status fail_hip(hipError_t e, const char *what)
+{
+    gpp_status st = e == hipErrorOutOfMemory ? GPP_ERR_OUT_OF_MEMORY
+                  : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? GPP_ERR_NO_DEVICE
+                                                                          : GPP_ERR_LAUNCH;
+    return fail(st, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_OR_FAIL(call, what)                         \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return fail_hip(e_, what); \
+    } while (0)
+
+// A device buffer that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap && ptr) return hipSuccess;
+        if (ptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipMalloc(&ptr, bytes ? bytes : 256);
+        if (e == hipSuccess) cap = bytes;
+        else ptr = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+gpp_status check_problem(const gpp_problem *p)
+{
+    if (!p) return fail(GPP_ERR_INVALID_ARGUMENT, "problem is NULL");
+    if (p->number_bands < 1 || p->nvband < 0 || p->nvband > p->number_bands || p->ncouls < 1 ||
+        p->ngpown < 1 || p->nfreq < 1 || p->nfreq > 4096)
+        return fail(GPP_ERR_INVALID_ARGUMENT,
+                    "invalid sizes: number_bands=%d nvband=%d ncouls=%d ngpown=%d nfreq=%d "
+                    "(need number_bands>=1, 0<=nvband<=number_bands, ncouls>=1, ngpown>=1, 1<=nfreq<=4096)",
+                    p->number_bands, p->nvband, p->ncouls, p->ngpown, p->nfreq);
+    if ((uint64_t)p->number_bands * (uint64_t)p->ncouls > (1ull << 40) ||
+        (uint64_t)p->ngpown * (uint64_t)p->ncouls > (1ull << 40))
+        return fail(GPP_ERR_INVALID_ARGUMENT, "problem too large");
+    return GPP_OK;
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+namespace gpp {
+
+hipError_t Workspace::reserve(size_t need)
+{
+    if (ptr && need <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&ptr, need ? need : 256);
+    if (e == hipSuccess) bytes = need;
+    else ptr = nullptr;
+    return e;
+}
+
+void Workspace::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+hipError_t StageTimer::init()
+{
+    for (int i = 0; i <= kMax; ++i) {
+        hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) return e;
+    }
+    ready = true;
+    return hipSuccess;
+}
+
+void StageTimer::destroy()
+{
+    if (!ready) return;
+    for (int i = 0; i <= kMax; ++i) (void)hipEventDestroy(ev[i]);
+    ready = false;
+}
+
+void StageTimer::begin(hipStream_t s)
+{
+    n = 0;
+    if (ready) (void)hipEventRecord(ev[0], s);
+}
+
+void StageTimer::mark(hipStream_t s, const char *nm)
+{
+    if (!ready || n >= kMax) return;
+    name[n] = nm;
+    (void)hipEventRecord(ev[n + 1], s);
+    ++n;
+}
+
+}  // namespace gpp
+
+// ---------------------------------------------------------------------------
+struct gpp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gpp_problem prob{};
+    bool have_inputs = false;
+    gpp::DevProblem dp;
+    DevBuf idx, wx, vcoul, aqsm, aqsn, ieps, wtilde;  // owned inputs
+    DevBuf results, flag;
+    gpp::Workspace work;
+    gpp::StageTimer timer;
+    bool timed = false;
+};
+
+namespace {
+
+gpp_status set_device(const gpp_ctx *ctx)
+{
+    HIP_OR_FAIL(hipSetDevice(ctx->device), "hipSetDevice");
+    return GPP_OK;
+}
+
+void fill_dp_sizes(gpp::DevProblem &dp, const gpp_problem *p)
+{
+    dp.nb = p->number_bands;
+    dp.nv = p->nvband;
+    dp.ncouls = p->ncouls;
+    dp.ngpown = p->ngpown;
+    dp.nfreq = p->nfreq;
+    dp.occ = p->occ;
+}
+
+gpp_status ensure_owned(gpp_ctx *ctx, const gpp_problem *p)
+{
+    const size_t nb = p->number_bands, nc = p->ncouls, ng = p->ngpown, nf = p->nfreq;
+    HIP_OR_FAIL(ctx->idx.ensure(sizeof(int32_t) * ng), "hipMalloc inv_igp_index");
+    HIP_OR_FAIL(ctx->wx.ensure(sizeof(double) * nf), "hipMalloc wx_array");
+    HIP_OR_FAIL(ctx->vcoul.ensure(sizeof(double) * nc), "hipMalloc vcoul");
+    HIP_OR_FAIL(ctx->aqsm.ensure(sizeof(cplx) * nb * nc), "hipMalloc aqsmtemp");
+    HIP_OR_FAIL(ctx->aqsn.ensure(sizeof(cplx) * nb * nc), "hipMalloc aqsntemp");
+    HIP_OR_FAIL(ctx->ieps.ensure(sizeof(cplx) * ng * nc), "hipMalloc I_eps_array");
+    HIP_OR_FAIL(ctx->wtilde.ensure(sizeof(cplx) * ng * nc), "hipMalloc wtilde_array");
+    gpp::DevProblem &dp = ctx->dp;
+    fill_dp_sizes(dp, p);
+    dp.idx = static_cast<const int32_t *>(ctx->idx.ptr);
+    dp.wx = static_cast<const double *>(ctx->wx.ptr);
+    dp.vcoul = static_cast<const double *>(ctx->vcoul.ptr);
+    dp.aqsm = static_cast<const cplx *>(ctx->aqsm.ptr);
+    dp.aqsn = static_cast<const cplx *>(ctx->aqsn.ptr);
+    dp.ieps = static_cast<const cplx *>(ctx->ieps.ptr);
+    dp.wtilde = static_cast<const cplx *>(ctx->wtilde.ptr);
+    return GPP_OK;
+}
+
+gpp_status check_index_device(gpp_ctx *ctx, const int32_t *d_idx, int ngpown)
+{
+    HIP_OR_FAIL(ctx->flag.ensure(sizeof(int)), "hipMalloc flag");
+    HIP_OR_FAIL(hipMemsetAsync(ctx->flag.ptr, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+    HIP_OR_FAIL(gpp::launch_check_index(d_idx, ngpown, static_cast<int *>(ctx->flag.ptr), ctx->stream),
+                "check_index launch");
+    int bad = 0;
+    HIP_OR_FAIL(hipMemcpyAsync(&bad, ctx->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+                "hipMemcpyAsync");
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (bad) return fail(GPP_ERR_BAD_INDEX, "%d negative inv_igp_index entries", bad);
+    return GPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gpp_last_error(void) { return g_err; }
+
+int gpp_abi_version(void) { return GPP_ABI_VERSION; }
+
+gpp_status gpp_ctx_create(gpp_ctx **out, int device)
+{
+    if (!out) return fail(GPP_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(GPP_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(GPP_ERR_NO_DEVICE, "device %d out of range [0,%d)", device, n);
+    hipDeviceProp_t prop;
+    HIP_OR_FAIL(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(GPP_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device,
+                    prop.gcnArchName);
+    gpp_ctx *ctx = new (std::nothrow) gpp_ctx();
+    if (!ctx) return fail(GPP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    ctx->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = ctx->timer.init();
+    if (e != hipSuccess) {
+        gpp_ctx_destroy(ctx);
+        return fail_hip(e, "context setup");
+    }
+    *out = ctx;
+    return GPP_OK;
+}
+
+gpp_status gpp_ctx_destroy(gpp_ctx *ctx)
+{
+    if (!ctx) return GPP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->idx.release(); ctx->wx.release(); ctx->vcoul.release();
+    ctx->aqsm.release(); ctx->aqsn.release(); ctx->ieps.release(); ctx->wtilde.release();
+    ctx->results.release(); ctx->flag.release();
+    ctx->work.release();
+    ctx->timer.destroy();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GPP_OK;
+}
+
+gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_igp_index,
+                      const double *wx_array, const double *vcoul, const gpp_complex *aqsmtemp,
+                      const gpp_complex *aqsntemp, const gpp_complex *I_eps_array,
+                      const gpp_complex *wtilde_array)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    gpp_status st = check_problem(prob);
+    if (st) return st;
+    if (!inv_igp_index || !wx_array || !vcoul || !aqsmtemp || !aqsntemp || !I_eps_array || !wtilde_array)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL input array");
+    for (int j = 0; j < prob->ngpown; ++j)
+        if (inv_igp_index[j] < 0)
+            return fail(GPP_ERR_BAD_INDEX, "inv_igp_index[%d] = %d is negative", j, inv_igp_index[j]);
+    if ((st = set_device(ctx))) return st;
+    ctx->have_inputs = false;
+    if ((st = ensure_owned(ctx, prob))) return st;
+    const size_t nb = prob->number_bands, nc = prob->ncouls, ng = prob->ngpown, nf = prob->nfreq;
+    hipStream_t s = ctx->stream;
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->idx.ptr, inv_igp_index, sizeof(int32_t) * ng, hipMemcpyHostToDevice, s), "H2D inv_igp_index");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->wx.ptr, wx_array, sizeof(double) * nf, hipMemcpyHostToDevice, s), "H2D wx_array");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->vcoul.ptr, vcoul, sizeof(double) * nc, hipMemcpyHostToDevice, s), "H2D vcoul");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->aqsm.ptr, aqsmtemp, sizeof(cplx) * nb * nc, hipMemcpyHostToDevice, s), "H2D aqsmtemp");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->aqsn.ptr, aqsntemp, sizeof(cplx) * nb * nc, hipMemcpyHostToDevice, s), "H2D aqsntemp");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->ieps.ptr, I_eps_array, sizeof(cplx) * ng * nc, hipMemcpyHostToDevice, s), "H2D I_eps_array");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->wtilde.ptr, wtilde_array, sizeof(cplx) * ng * nc, hipMemcpyHostToDevice, s), "H2D wtilde_array");
+    HIP_OR_FAIL(hipStreamSynchronize(s), "upload synchronize");
+    ctx->prob = *prob;
+    ctx->have_inputs = true;
+    return GPP_OK;
+}
+
+gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *d_inv_igp_index,
+                           const double *d_wx_array, const double *d_vcoul,
+                           const gpp_complex *d_aqsmtemp, const gpp_complex *d_aqsntemp,
+                           const gpp_complex *d_I_eps_array, const gpp_complex *d_wtilde_array)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    gpp_status st = check_problem(prob);
+    if (st) return st;
+    if (!d_inv_igp_index || !d_wx_array || !d_vcoul || !d_aqsmtemp || !d_aqsntemp || !d_I_eps_array ||
+        !d_wtilde_array)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL device array");
+    if (!aligned16(d_aqsmtemp) || !aligned16(d_aqsntemp) || !aligned16(d_I_eps_array) || !aligned16(d_wtilde_array))
+        return fail(GPP_ERR_INVALID_ARGUMENT, "complex device arrays must be 16-byte aligned");
+    if ((st = set_device(ctx))) return st;
+    ctx->have_inputs = false;
+    if ((st = check_index_device(ctx, d_inv_igp_index, prob->ngpown))) return st;
+    gpp::DevProblem &dp = ctx->dp;
+    fill_dp_sizes(dp, prob);
+    dp.idx = d_inv_igp_index;
+    dp.wx = d_wx_array;
+    dp.vcoul = d_vcoul;
+    dp.aqsm = reinterpret_cast<const cplx *>(d_aqsmtemp);
+    dp.aqsn = reinterpret_cast<const cplx *>(d_aqsntemp);
+    dp.ieps = reinterpret_cast<const cplx *>(d_I_eps_array);
+    dp.wtilde = reinterpret_cast<const cplx *>(d_wtilde_array);
+    ctx->prob = *prob;
+    ctx->have_inputs = true;
+    return GPP_OK;
+}
+
+gpp_status gpp_fill_deterministic(const gpp_problem *prob, int igp_zero_based, int32_t *inv_igp_index,
+                                  double *wx_array, double *vcoul, gpp_complex *aqsmtemp,
+                                  gpp_complex *aqsntemp, gpp_complex *I_eps_array,
+                                  gpp_complex *wtilde_array)
+{
+    gpp_status st = check_problem(prob);
+    if (st) return st;
+    const int nb = prob->number_bands, nc = prob->ncouls, ng = prob->ngpown, nf = prob->nfreq;
+    // gppComplex.cpp:239-247
+    const double e_lk = 10, dw = 1, to1 = 1e-6, e_n1kq = 6.0;
+    // expr = (0.5, 0.5); GPPComplex_mult(expr, (double)(i+j)), :293, :296-310
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nc; j++) {
+            const double v = 0.5 * (double)(i + j);
+            const size_t o = (size_t)i * nc + j;
+            if (aqsntemp) aqsntemp[o] = gpp_complex{v, v};
+            if (aqsmtemp) aqsmtemp[o] = gpp_complex{v, v};
+        }
+    for (int i = 0; i < ng; i++) {
+        for (int j = 0; j < nc; j++) {
+            const double v = 0.5 * (double)(i + j);
+            const size_t o = (size_t)i * nc + j;
+            if (I_eps_array) I_eps_array[o] = gpp_complex{v, v};
+            if (wtilde_array) wtilde_array[o] = gpp_complex{v, v};
+        }
+        // :312
+        if (inv_igp_index)
+            inv_igp_index[i] = (int32_t)(((int64_t)(igp_zero_based ? i : i + 1) * nc) / ng);
+    }
+    if (vcoul)
+        for (int i = 0; i < nc; i++) vcoul[i] = 1.0 * i;  // :319
+    if (wx_array)
+        for (int iw = 0; iw < nf; ++iw) {
+            wx_array[iw] = e_lk - e_n1kq + dw * ((iw + 1) - 2);  // :327
+            if (wx_array[iw] < to1) wx_array[iw] = to1;          // :328
+        }
+    return GPP_OK;
+}
+
+gpp_status gpp_fill_seeded(const gpp_problem *prob, uint64_t seed, double wx_center, double dw,
+                           int32_t *inv_igp_index, double *wx_array, double *vcoul, gpp_complex *aqsmtemp,
+                           gpp_complex *aqsntemp, gpp_complex *I_eps_array, gpp_complex *wtilde_array)
+{
+    gpp_status st = check_problem(prob);
+    if (st) return st;
+    const size_t nb = prob->number_bands, nc = prob->ncouls, ng = prob->ngpown;
+    if (ng > nc) return fail(GPP_ERR_INVALID_ARGUMENT, "seeded inputs need ngpown <= ncouls");
+    auto fill = [&](gpp_complex *dst, size_t n, uint32_t id, double rlo, double rhi, double ilo, double ihi) {
+        if (!dst) return;
+        for (size_t i = 0; i < n; ++i) {
+            dst[i].re = gpp::seeded_uniform(seed, id, 2 * i, rlo, rhi);
+            dst[i].im = gpp::seeded_uniform(seed, id, 2 * i + 1, ilo, ihi);
+        }
+    };
+    fill(aqsmtemp, nb * nc, gpp::kArrAqsm, -1.0, 1.0, -1.0, 1.0);
+    fill(aqsntemp, nb * nc, gpp::kArrAqsn, -1.0, 1.0, -1.0, 1.0);
+    fill(I_eps_array, ng * nc, gpp::kArrIeps, -0.5, 0.5, -0.5, 0.5);
+    fill(wtilde_array, ng * nc, gpp::kArrWtilde, 0.5, 5.0, -0.05, 0.05);
+    if (vcoul)
+        for (size_t i = 0; i < nc; ++i) vcoul[i] = gpp::seeded_uniform(seed, gpp::kArrVcoul, i, 0.1, 10.0);
+    if (inv_igp_index) {
+        // Selection sampling (Knuth, TAOCP vol. 2, 3.4.2, Algorithm S): each
+        // index is kept with probability needed/remaining; output is sorted.
+        size_t need = ng, out = 0;
+        for (size_t i = 0; i < nc && need > 0; ++i) {
+            const double u = gpp::seeded_u01(seed, gpp::kArrIndex, i);
+            if (u * (double)(nc - i) < (double)need) {
+                inv_igp_index[out++] = (int32_t)i;
+                --need;
+            }
+        }
+    }
+    if (wx_array)
+        for (int iw = 0; iw < prob->nfreq; ++iw) {
+            wx_array[iw] = wx_center + dw * (iw - 0.5 * (prob->nfreq - 1));
+            if (wx_array[iw] < 1e-6) wx_array[iw] = 1e-6;
+        }
+    return GPP_OK;
+}
+
+gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t seed, double wx_center,
+                               double dw)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    gpp_status st = check_problem(prob);
+    if (st) return st;
+    if (prob->ngpown > prob->ncouls) return fail(GPP_ERR_INVALID_ARGUMENT, "seeded inputs need ngpown <= ncouls");
+    if ((st = set_device(ctx))) return st;
+    ctx->have_inputs = false;
+    if ((st = ensure_owned(ctx, prob))) return st;
+    std::vector<int32_t> idx(prob->ngpown);
+    std::vector<double> wx(prob->nfreq), vc(prob->ncouls);
+    st = gpp_fill_seeded(prob, seed, wx_center, dw, idx.data(), wx.data(), vc.data(), nullptr, nullptr,
+                         nullptr, nullptr);
+    if (st) return st;
+    const size_t nb = prob->number_bands, nc = prob->ncouls, ng = prob->ngpown;
+    hipStream_t s = ctx->stream;
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->idx.ptr, idx.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, s), "H2D idx");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->wx.ptr, wx.data(), sizeof(double) * prob->nfreq, hipMemcpyHostToDevice, s), "H2D wx");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->vcoul.ptr, vc.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s), "H2D vcoul");
+    HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->aqsm.ptr), nb * nc, seed, gpp::kArrAqsm, -1.0, 1.0, -1.0, 1.0, s), "fill aqsm");
+    HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->aqsn.ptr), nb * nc, seed, gpp::kArrAqsn, -1.0, 1.0, -1.0, 1.0, s), "fill aqsn");
+    HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->ieps.ptr), ng * nc, seed, gpp::kArrIeps, -0.5, 0.5, -0.5, 0.5, s), "fill I_eps");
+    HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->wtilde.ptr), ng * nc, seed, gpp::kArrWtilde, 0.5, 5.0, -0.05, 0.05, s), "fill wtilde");
+    HIP_OR_FAIL(hipStreamSynchronize(s), "generate synchronize");
+    ctx->prob = *prob;
+    ctx->have_inputs = true;
+    return GPP_OK;
+}
+
+gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out || !out->achtemp || !out->asxtemp)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
+    if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    const gpp::DevProblem &dp = ctx->dp;
+    size_t need = 0;
+    switch (form) {
+    case GPP_FORM_DIRECT:
+    case GPP_FORM_DIRECT_STRICT: need = gpp::direct_workspace_bytes(dp); break;
+    case GPP_FORM_FACTORED_F64: need = gpp::factored_workspace_bytes(dp, false); break;
+    case GPP_FORM_FACTORED_OZAKI: need = gpp::factored_workspace_bytes(dp, true); break;
+    default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
+    }
+    if (need == 0) return fail(GPP_ERR_UNSUPPORTED, "form %d is not available for this problem", (int)form);
+    HIP_OR_FAIL(ctx->work.reserve(need), "workspace allocation");
+    gpp::DevResults res;
+    res.nfreq = dp.nfreq;
+    res.nb = dp.nb;
+    HIP_OR_FAIL(ctx->results.ensure(sizeof(cplx) * res.count()), "results allocation");
+    res.base = static_cast<cplx *>(ctx->results.ptr);
+    hipError_t e;
+    if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT)
+        e = gpp::run_direct(dp, form == GPP_FORM_DIRECT_STRICT, ctx->work.ptr, res, ctx->stream, &ctx->timer);
+    else
+        e = gpp::run_factored(dp, form == GPP_FORM_FACTORED_OZAKI, ctx->work.ptr, res, ctx->stream, &ctx->timer);
+    if (e != hipSuccess) return fail_hip(e, "sigma launch");
+    std::vector<cplx> host(res.count());
+    HIP_OR_FAIL(hipMemcpyAsync(host.data(), res.base, sizeof(cplx) * res.count(), hipMemcpyDeviceToHost, ctx->stream),
+                "D2H results");
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "sigma synchronize");
+    ctx->timed = true;
+    memcpy(out->achtemp, host.data(), sizeof(cplx) * dp.nfreq);
+    memcpy(out->asxtemp, host.data() + dp.nfreq, sizeof(cplx) * dp.nfreq);
+    if (out->acht_n1_loc) memcpy(out->acht_n1_loc, host.data() + 2 * dp.nfreq, sizeof(cplx) * dp.nb);
+    if (out->achstemp) memcpy(out->achstemp, host.data() + 2 * dp.nfreq + dp.nb, sizeof(cplx));
+    return GPP_OK;
+}
+
+gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *stage_ms, int max_stages,
+                               int *nstages)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    const gpp::StageTimer &t = ctx->timer;
+    if (!ctx->timed || t.n == 0) {
+        if (total_ms) *total_ms = 0.0;
+        if (nstages) *nstages = 0;
+        return GPP_OK;
+    }
+    HIP_OR_FAIL(hipEventSynchronize(t.ev[t.n]), "hipEventSynchronize");
+    float ms = 0.f;
+    HIP_OR_FAIL(hipEventElapsedTime(&ms, t.ev[0], t.ev[t.n]), "hipEventElapsedTime");
+    if (total_ms) *total_ms = ms;
+    for (int i = 0; i < t.n && stage_ms && i < max_stages; ++i) {
+        HIP_OR_FAIL(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]), "hipEventElapsedTime");
+        stage_ms[i] = ms;
+    }
+    if (nstages) *nstages = t.n;
+    return GPP_OK;
+}
+
+const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i)
+{
+    if (!ctx || i < 0 || i >= ctx->timer.n || !ctx->timer.name[i]) return "";
+    return ctx->timer.name[i];
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference-compatible launchers.  They share one grow-only scratch area per
+// device (partials, results, index flag), guarded by a mutex: calls from
+// several host threads serialise on it.
+namespace {
+
+struct CompatScratch {
+    gpp::Workspace work, results, flag;
+};
+
+std::mutex g_compat_mu;
+std::vector<CompatScratch> g_compat;
+
+gpp_status compat_run(int which, const gpp::DevProblem &dp, double *ach_re, double *ach_im, cplx *asx,
+                      cplx *acht_n1, double *achs_re, double *achs_im)
+{
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    int dev = 0;
+    HIP_OR_FAIL(hipGetDevice(&dev), "hipGetDevice");
+    if ((size_t)dev >= g_compat.size()) g_compat.resize(dev + 1);
+    CompatScratch &cs = g_compat[dev];
+    hipStream_t s = nullptr;  // null stream, as the reference launches
+    HIP_OR_FAIL(cs.flag.reserve(sizeof(int)), "hipMalloc flag");
+    HIP_OR_FAIL(hipMemsetAsync(cs.flag.ptr, 0, sizeof(int), s), "hipMemsetAsync");
+    HIP_OR_FAIL(gpp::launch_check_index(dp.idx, dp.ngpown, static_cast<int *>(cs.flag.ptr), s), "check_index");
+    int bad = 0;
+    HIP_OR_FAIL(hipMemcpy(&bad, cs.flag.ptr, sizeof(int), hipMemcpyDeviceToHost), "D2H flag");
+    if (bad) return fail(GPP_ERR_BAD_INDEX, "%d negative inv_igp_index entries", bad);
+    HIP_OR_FAIL(cs.work.reserve(gpp::direct_workspace_bytes(dp)), "workspace allocation");
+    gpp::DevResults res;
+    res.nfreq = dp.nfreq;
+    res.nb = dp.nb;
+    HIP_OR_FAIL(cs.results.reserve(sizeof(cplx) * res.count()), "results allocation");
+    res.base = static_cast<cplx *>(cs.results.ptr);
+    HIP_OR_FAIL(gpp::run_direct_part(dp, which, cs.work.ptr, res, s), "direct launch");
+    HIP_OR_FAIL(gpp::launch_accumulate_compat(res, which, dp.nv, ach_re, ach_im, asx, acht_n1, achs_re, achs_im, s),
+                "accumulate launch");
+    return GPP_OK;
+}
+
+gpp::DevProblem compat_problem(int nb, int nv, int ncouls, int ngpown, const int *idx, const double *wx,
+                               const double *vcoul, const gpp_complex *aqsm, const gpp_complex *aqsn,
+                               const gpp_complex *ieps, const gpp_complex *wtilde, double occ)
+{
+    gpp::DevProblem dp;
+    dp.nb = nb; dp.nv = nv; dp.ncouls = ncouls; dp.ngpown = ngpown; dp.nfreq = 3; dp.occ = occ;
+    dp.idx = idx; dp.wx = wx; dp.vcoul = vcoul;
+    dp.aqsm = reinterpret_cast<const cplx *>(aqsm);
+    dp.aqsn = reinterpret_cast<const cplx *>(aqsn);
+    dp.ieps = reinterpret_cast<const cplx *>(ieps);
+    dp.wtilde = reinterpret_cast<const cplx *>(wtilde);
+    return dp;
+}
+
+}  // namespace
+
+extern "C" {
+
+gpp_status d_till_nvband(int nvband, int ngpown, int ncouls, int *inv_igp_index, double *wx_array,
+                         gpp_complex *wtilde_array, gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                         gpp_complex *I_eps_array, gpp_complex *asxtemp, double *vcoul,
+                         gpp_complex *acht_n1_loc, double *achtemp_re, double *achtemp_im, const double occ)
+{
+    if (nvband < 0 || ngpown < 1 || ncouls < 1)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "invalid sizes nvband=%d ngpown=%d ncouls=%d", nvband, ngpown, ncouls);
+    if (nvband == 0) return GPP_OK;
+    if (!inv_igp_index || !wx_array || !wtilde_array || !aqsmtemp || !aqsntemp || !I_eps_array || !vcoul ||
+        !achtemp_re || !achtemp_im)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL device array");
+    gpp::DevProblem dp = compat_problem(nvband, nvband, ncouls, ngpown, inv_igp_index, wx_array, vcoul, aqsmtemp,
+                                        aqsntemp, I_eps_array, wtilde_array, occ);
+    return compat_run(0, dp, achtemp_re, achtemp_im, reinterpret_cast<cplx *>(asxtemp),
+                      reinterpret_cast<cplx *>(acht_n1_loc), nullptr, nullptr);
+}
+
+gpp_status d_reduce_achstemp(int number_bands, int *inv_igp_index, int ncouls, gpp_complex *aqsmtemp,
+                             gpp_complex *aqsntemp, gpp_complex *I_eps_array, double *achstemp_re,
+                             double *achstemp_im, int ngpown, double *vcoul)
+{
+    if (number_bands < 1 || ngpown < 1 || ncouls < 1)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "invalid sizes number_bands=%d ngpown=%d ncouls=%d", number_bands,
+                    ngpown, ncouls);
+    if (!inv_igp_index || !aqsmtemp || !aqsntemp || !I_eps_array || !vcoul || !achstemp_re || !achstemp_im)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL device array");
+    gpp::DevProblem dp = compat_problem(number_bands, 0, ncouls, ngpown, inv_igp_index, nullptr, vcoul, aqsmtemp,
+                                        aqsntemp, I_eps_array, nullptr, 1.0);
+    return compat_run(2, dp, nullptr, nullptr, nullptr, nullptr, achstemp_re, achstemp_im);
+}
+
+gpp_status d_noFlagOCCSolver(int number_bands, int nvband, int ngpown, int ncouls, int *inv_igp_index,
+                             double *wx_array, gpp_complex *wtilde_array, gpp_complex *aqsmtemp,
+                             gpp_complex *aqsntemp, gpp_complex *I_eps_array, double *vcoul, double *achtemp_re,
+                             double *achtemp_im, gpp_complex *acht_n1_loc)
+{
+    if (number_bands < 1 || nvband < 0 || nvband > number_bands || ngpown < 1 || ncouls < 1)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "invalid sizes number_bands=%d nvband=%d ngpown=%d ncouls=%d",
+                    number_bands, nvband, ngpown, ncouls);
+    if (nvband == number_bands) return GPP_OK;
+    if (!inv_igp_index || !wx_array || !wtilde_array || !aqsmtemp || !aqsntemp || !I_eps_array || !vcoul ||
+        !achtemp_re || !achtemp_im)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL device array");
+    gpp::DevProblem dp = compat_problem(number_bands, nvband, ncouls, ngpown, inv_igp_index, wx_array, vcoul,
+                                        aqsmtemp, aqsntemp, I_eps_array, wtilde_array, 1.0);
+    return compat_run(1, dp, achtemp_re, achtemp_im, nullptr, reinterpret_cast<cplx *>(acht_n1_loc), nullptr,
+                      nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// gpp_device.hpp -- device-side complex type, reference-order complex helpers
+// and deterministic wavefront / block reductions shared by the GPP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpp {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// 16-byte interleaved complex, layout-identical to the reference's GPPComplex
+// (Complex.h:15-19) and to gpp_complex of include/gpp.h.
+struct alignas(16) cplx {
+    double re;
+    double im;
+};
+
+__host__ __device__ __forceinline__ cplx mk(double re, double im) { return cplx{re, im}; }
+__host__ __device__ __forceinline__ cplx czero() { return cplx{0.0, 0.0}; }
+
+// The helpers below follow the reference's operation order and are compiled
+// without FMA contraction (the pragma sits in each body because it does not
+// reach inlined callees), so each rounds exactly as the x86-64 reference does.
+
+// GPPComplex_product, Complex.h:167-174.
+__device__ __forceinline__ cplx cmul(cplx a, cplx b)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// GPPComplex_conj, Complex.h:153-161.
+__device__ __forceinline__ cplx cconj(cplx a) { return cplx{a.re, -1 * a.im}; }
+// GPPComplex_square, Complex.h:141-148.
+__device__ __forceinline__ cplx csquare(cplx a)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re * a.re - a.im * a.im, 2 * a.re * a.im};
+}
+// GPPComplex_mult(a, b), Complex.h:200-205.
+__device__ __forceinline__ cplx cscale(cplx a, double b) { return cplx{a.re * b, a.im * b}; }
+// GPPComplex_mult(a, b, c), Complex.h:190-195: ((re*b)*c, (im*b)*c).
+__device__ __forceinline__ cplx cscale2(cplx a, double b, double c) { return cplx{a.re * b * c, a.im * b * c}; }
+// doubleMinusGPPComplex, Complex.h:234-237.
+__device__ __forceinline__ cplx cdminus(double a, cplx z) { return cplx{a - z.re, 0 - z.im}; }
+// doublePlusGPPComplex, Complex.h:239-242.
+__device__ __forceinline__ cplx cdplus(double a, cplx z) { return cplx{a + z.re, 0 + z.im}; }
+// GPPComplex_abs, Complex.h:179-185 (with sqrt).
+__device__ __forceinline__ double cabs_(cplx a)
+{
+#pragma clang fp contract(off)
+    return sqrt(a.re * a.re + a.im * a.im);
+}
+// operator+=, Complex.h:51-56: dst = src + dst.
+__device__ __forceinline__ void cacc(cplx &dst, cplx src)
+{
+    dst.re = src.re + dst.re;
+    dst.im = src.im + dst.im;
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+
+// 1/d for d > 0 to within about one ulp: the hardware estimate refined by two
+// Newton-Raphson steps (each at least doubles the correct bits).  d = 0 gives
+// NaN, as the reference's 1/0 = inf times the zero numerator it multiplies;
+// NaN propagates.  Valid for d in [2^-1000, 2^1000]; beyond that range the
+// result is NaN (callers needing IEEE behaviour there use the strict form).
+__device__ __forceinline__ double rcp_refined(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return r;
+}
+
+// Product a*b with FMA contraction allowed (fast paths only).
+__device__ __forceinline__ cplx cmul_fast(cplx a, cplx b)
+{
+    return cplx{__builtin_fma(a.re, b.re, -a.im * b.im), __builtin_fma(a.re, b.im, a.im * b.re)};
+}
+
+// Wave-wide sum by xor butterfly.  Every lane ends with the same, bitwise
+// reproducible value (a+b == b+a in IEEE arithmetic).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+__device__ __forceinline__ cplx wave_sum(cplx v) { return cplx{wave_sum(v.re), wave_sum(v.im)}; }
+
+// Block-wide sum in a fixed order: butterfly within each wave, then the wave
+// totals in wave order.  Every thread of the block returns the same value.
+// `scratch` needs blockDim.x / 64 entries; the call contains two barriers.
+__device__ __forceinline__ cplx block_sum(cplx v, cplx *scratch)
+{
+    const int nw = blockDim.x / kWave;
+    const int w = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if ((threadIdx.x % kWave) == 0) scratch[w] = v;
+    __syncthreads();
+    cplx r = czero();
+    for (int i = 0; i < nw; ++i) r = cadd(r, scratch[i]);
+    __syncthreads();
+    return r;
+}
+
+// igp = min(inv_igp_index[j], ncouls-1), gppComplex.cpp:342-344.  Negative
+// entries are rejected before any launch; the max(.,0) only keeps a kernel
+// in bounds should one slip through.
+__device__ __forceinline__ int igp_of(const int32_t *idx, int j, int ncouls)
+{
+    int igp = idx[j];
+    igp = igp >= ncouls ? ncouls - 1 : igp;
+    return igp < 0 ? 0 : igp;
+}
+
+}  // namespace gpp

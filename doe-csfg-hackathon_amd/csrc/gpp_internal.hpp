@@ -1,0 +1,94 @@
+// gpp_internal.hpp -- host-side interfaces shared between the C ABI layer
+// (gpp_api.cpp) and the kernel launchers (gpp_direct.hip, gpp_factored.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gpp_device.hpp"
+
+namespace gpp {
+
+// One problem's sizes and device inputs (row-major, see include/gpp.h).
+struct DevProblem {
+    int nb = 0, nv = 0, ncouls = 0, ngpown = 0, nfreq = 0;
+    double occ = 1.0;
+    const int32_t *idx = nullptr;  // [ngpown]
+    const double *wx = nullptr;    // [nfreq]
+    const double *vcoul = nullptr; // [ncouls]
+    const cplx *aqsm = nullptr;    // [nb][ncouls]
+    const cplx *aqsn = nullptr;    // [nb][ncouls]
+    const cplx *ieps = nullptr;    // [ngpown][ncouls]
+    const cplx *wtilde = nullptr;  // [ngpown][ncouls]
+};
+
+// Device results of one evaluation, contiguous:
+//   achtemp[nfreq], asxtemp[nfreq], acht_n1_loc[nb], achstemp[1].
+struct DevResults {
+    cplx *base = nullptr;
+    int nfreq = 0, nb = 0;
+    cplx *achtemp() const { return base; }
+    cplx *asxtemp() const { return base + nfreq; }
+    cplx *acht_n1_loc() const { return base + 2 * nfreq; }
+    cplx *achstemp() const { return base + 2 * nfreq + nb; }
+    size_t count() const { return (size_t)2 * nfreq + nb + 1; }
+};
+
+// Grow-only device scratch owned by a context (or by the reference-
+// compatible launchers' per-device cache).
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+    hipError_t reserve(size_t need);
+    void release();
+};
+
+// Records HIP events between the stages of one evaluation.
+struct StageTimer {
+    static constexpr int kMax = 12;
+    hipEvent_t ev[kMax + 1] = {};
+    const char *name[kMax] = {};
+    int n = 0;        // stages recorded in the current evaluation
+    bool ready = false;
+    hipError_t init();
+    void destroy();
+    void begin(hipStream_t s);                 // records ev[0]
+    void mark(hipStream_t s, const char *nm);  // closes one stage
+};
+
+// Bytes of device workspace the direct form needs for this problem.
+size_t direct_workspace_bytes(const DevProblem &p);
+
+// Direct FP64 evaluation (strict = reference operation order).  Writes the
+// results to `res` (overwritten).  Workspace must hold
+// direct_workspace_bytes(p).
+hipError_t run_direct(const DevProblem &p, bool strict, void *work, const DevResults &res,
+                      hipStream_t s, StageTimer *timer);
+
+// Only the occupied bands (n1 < nv) / only the unoccupied bands of the direct
+// form, plus achstemp, for the reference-compatible launchers.  `which`
+// selects 0 = occupied, 1 = unoccupied, 2 = achstemp.
+hipError_t run_direct_part(const DevProblem &p, int which, void *work, const DevResults &res,
+                           hipStream_t s);
+
+// Counts negative inv_igp_index entries on the device (result in *d_count).
+hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hipStream_t s);
+
+// Adds res (nfreq=3 layout) into reference-style split outputs.
+hipError_t launch_accumulate_compat(const DevResults &res, int which, int nv, double *achtemp_re,
+                                    double *achtemp_im, cplx *asxtemp, cplx *acht_n1_loc,
+                                    double *achstemp_re, double *achstemp_im, hipStream_t s);
+
+// Seeded input generation on the device (counter-based hash, identical bits
+// to the host generator).
+hipError_t launch_fill_seeded(cplx *dst, size_t n, uint64_t seed, uint32_t array_id, double re_lo,
+                              double re_hi, double im_lo, double im_hi, hipStream_t s);
+
+// Factored forms.
+size_t factored_workspace_bytes(const DevProblem &p, bool ozaki);
+hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevResults &res,
+                        hipStream_t s, StageTimer *timer);
+
+}  // namespace gpp

@@ -1,0 +1,267 @@
+"""ctypes binding of libgpp_hip.so, the C ABI declared in include/gpp.h.
+
+Used by the tests, bench.py and the smoke entry point to drive the HIP
+library exactly as a C caller would.  There is no fallback: if the built
+library is missing, :func:`load` raises.
+
+Arrays follow the reference mini-app's layout (gppComplex.cpp:264-271):
+complex arrays are interleaved (re, im) float64 -- numpy ``complex128`` --
+row-major, aqsmtemp/aqsntemp ``[number_bands, ncouls]``, I_eps_array /
+wtilde_array ``[ngpown, ncouls]``, vcoul ``[ncouls]`` float64,
+inv_igp_index ``[ngpown]`` int32, wx_array ``[nfreq]`` float64.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libgpp_hip.so")
+
+# gpp_status, include/gpp.h
+GPP_OK = 0
+GPP_ERR_INVALID_ARGUMENT = 1
+GPP_ERR_BAD_INDEX = 2
+GPP_ERR_NO_DEVICE = 3
+GPP_ERR_OUT_OF_MEMORY = 4
+GPP_ERR_LAUNCH = 5
+GPP_ERR_UNSUPPORTED = 6
+GPP_ERR_NO_INPUTS = 7
+
+# gpp_form
+FORMS = {
+    "direct": 0,
+    "direct-strict": 1,
+    "factored-f64": 2,
+    "factored-ozaki": 3,
+}
+
+# Every symbol include/gpp.h declares.
+EXPORTED_SYMBOLS = (
+    "gpp_last_error",
+    "gpp_abi_version",
+    "gpp_ctx_create",
+    "gpp_ctx_destroy",
+    "gpp_upload",
+    "gpp_bind_device",
+    "gpp_fill_deterministic",
+    "gpp_fill_seeded",
+    "gpp_generate_seeded",
+    "gpp_sigma",
+    "gpp_ctx_last_timing",
+    "gpp_ctx_stage_name",
+    "d_till_nvband",
+    "d_reduce_achstemp",
+    "d_noFlagOCCSolver",
+)
+
+
+class GppProblem(ctypes.Structure):
+    _fields_ = [
+        ("number_bands", ctypes.c_int32),
+        ("nvband", ctypes.c_int32),
+        ("ncouls", ctypes.c_int32),
+        ("ngpown", ctypes.c_int32),
+        ("nfreq", ctypes.c_int32),
+        ("occ", ctypes.c_double),
+    ]
+
+
+class GppOutputs(ctypes.Structure):
+    _fields_ = [
+        ("achtemp", ctypes.c_void_p),
+        ("asxtemp", ctypes.c_void_p),
+        ("acht_n1_loc", ctypes.c_void_p),
+        ("achstemp", ctypes.c_void_p),
+    ]
+
+
+class GppError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"gpp status {status}: {message}")
+        self.status = status
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libgpp_hip.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    lib.gpp_last_error.restype = ctypes.c_char_p
+    lib.gpp_last_error.argtypes = []
+    lib.gpp_abi_version.restype = i32
+    lib.gpp_ctx_create.argtypes = [P(vp), i32]
+    lib.gpp_ctx_destroy.argtypes = [vp]
+    lib.gpp_upload.argtypes = [vp, P(GppProblem)] + [vp] * 7
+    lib.gpp_bind_device.argtypes = [vp, P(GppProblem)] + [vp] * 7
+    lib.gpp_fill_deterministic.argtypes = [P(GppProblem), i32] + [vp] * 7
+    lib.gpp_fill_seeded.argtypes = [P(GppProblem), ctypes.c_uint64, dbl, dbl] + [vp] * 7
+    lib.gpp_generate_seeded.argtypes = [vp, P(GppProblem), ctypes.c_uint64, dbl, dbl]
+    lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
+    lib.gpp_ctx_last_timing.argtypes = [vp, P(dbl), P(dbl), i32, P(i32)]
+    lib.gpp_ctx_stage_name.restype = ctypes.c_char_p
+    lib.gpp_ctx_stage_name.argtypes = [vp, i32]
+    lib.d_till_nvband.argtypes = [i32, i32, i32] + [vp] * 11 + [dbl]
+    lib.d_reduce_achstemp.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, i32, vp]
+    lib.d_noFlagOCCSolver.argtypes = [i32, i32, i32, i32] + [vp] * 10
+    for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device",
+                 "gpp_fill_deterministic", "gpp_fill_seeded", "gpp_generate_seeded", "gpp_sigma",
+                 "gpp_ctx_last_timing", "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
+        getattr(lib, name).restype = i32
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != GPP_OK:
+        raise GppError(status, load().gpp_last_error().decode())
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Problem:
+    number_bands: int
+    nvband: int
+    ncouls: int
+    ngpown: int
+    nfreq: int = 3
+    occ: float = 1.0
+
+    def c(self) -> GppProblem:
+        return GppProblem(self.number_bands, self.nvband, self.ncouls, self.ngpown, self.nfreq, self.occ)
+
+    @property
+    def updates(self) -> int:
+        """GPP sigma updates, one per (n1, igp, ig, iw)."""
+        return self.number_bands * self.ngpown * self.ncouls * self.nfreq
+
+
+def empty_inputs(p: Problem) -> Dict[str, np.ndarray]:
+    return {
+        "inv_igp_index": np.zeros(p.ngpown, np.int32),
+        "wx_array": np.zeros(p.nfreq, np.float64),
+        "vcoul": np.zeros(p.ncouls, np.float64),
+        "aqsmtemp": np.zeros((p.number_bands, p.ncouls), np.complex128),
+        "aqsntemp": np.zeros((p.number_bands, p.ncouls), np.complex128),
+        "I_eps_array": np.zeros((p.ngpown, p.ncouls), np.complex128),
+        "wtilde_array": np.zeros((p.ngpown, p.ncouls), np.complex128),
+    }
+
+
+_INPUT_ORDER = ("inv_igp_index", "wx_array", "vcoul", "aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array")
+
+
+def fill_deterministic(p: Problem, igp_zero_based: bool = False) -> Dict[str, np.ndarray]:
+    """The reference's deterministic inputs (gppComplex.cpp:296-329)."""
+    arrs = empty_inputs(p)
+    prob = p.c()
+    _check(load().gpp_fill_deterministic(ctypes.byref(prob), int(igp_zero_based),
+                                         *[_ptr(arrs[k]) for k in _INPUT_ORDER]))
+    return arrs
+
+
+def fill_seeded(p: Problem, seed: int, wx_center: float = 4.0, dw: float = 1.0) -> Dict[str, np.ndarray]:
+    """Seeded inputs with the benchmark distributions (see gpp_fill_seeded)."""
+    arrs = empty_inputs(p)
+    prob = p.c()
+    _check(load().gpp_fill_seeded(ctypes.byref(prob), seed, wx_center, dw,
+                                  *[_ptr(arrs[k]) for k in _INPUT_ORDER]))
+    return arrs
+
+
+def _contig(a: np.ndarray, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """A gpp_ctx on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = ctypes.c_void_p()
+        _check(lib.gpp_ctx_create(ctypes.byref(h), device))
+        self._h = h
+        self.problem: Optional[Problem] = None
+
+    def close(self) -> None:
+        if self._h:
+            load().gpp_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, p: Problem, arrs: Dict[str, np.ndarray]) -> None:
+        a = {
+            "inv_igp_index": _contig(arrs["inv_igp_index"], np.int32),
+            "wx_array": _contig(arrs["wx_array"], np.float64),
+            "vcoul": _contig(arrs["vcoul"], np.float64),
+        }
+        for k in ("aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array"):
+            a[k] = _contig(arrs[k], np.complex128)
+        prob = p.c()
+        _check(load().gpp_upload(self._h, ctypes.byref(prob), *[_ptr(a[k]) for k in _INPUT_ORDER]))
+        self.problem = p
+
+    def bind_device(self, p: Problem, ptrs: Dict[str, int]) -> None:
+        """Binds caller-owned device buffers (integer device addresses)."""
+        prob = p.c()
+        _check(load().gpp_bind_device(self._h, ctypes.byref(prob), *[ptrs[k] for k in _INPUT_ORDER]))
+        self.problem = p
+
+    def generate_seeded(self, p: Problem, seed: int, wx_center: float = 4.0, dw: float = 1.0) -> None:
+        prob = p.c()
+        _check(load().gpp_generate_seeded(self._h, ctypes.byref(prob), seed, wx_center, dw))
+        self.problem = p
+
+    def sigma(self, form: str = "direct") -> Dict[str, np.ndarray]:
+        if self.problem is None:
+            raise GppError(GPP_ERR_NO_INPUTS, "no inputs")
+        p = self.problem
+        out = {
+            "achtemp": np.zeros(p.nfreq, np.complex128),
+            "asxtemp": np.zeros(p.nfreq, np.complex128),
+            "acht_n1_loc": np.zeros(p.number_bands, np.complex128),
+            "achstemp": np.zeros(1, np.complex128),
+        }
+        o = GppOutputs(*[_ptr(out[k]) for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")])
+        _check(load().gpp_sigma(self._h, FORMS[form], ctypes.byref(o)))
+        out["achstemp"] = out["achstemp"][0]
+        return out
+
+    def last_timing(self) -> Dict[str, float]:
+        """GPU milliseconds of the last sigma: 'total' and one entry per stage."""
+        total = ctypes.c_double()
+        stages = (ctypes.c_double * 16)()
+        n = ctypes.c_int()
+        lib = load()
+        _check(lib.gpp_ctx_last_timing(self._h, ctypes.byref(total), stages, 16, ctypes.byref(n)))
+        res = {"total": total.value}
+        for i in range(n.value):
+            res[lib.gpp_ctx_stage_name(self._h, i).decode()] = stages[i]
+        return res

@@ -1,0 +1,243 @@
+/*
+ * gpp.h -- C ABI of the MI355X (gfx950) GPP sigma library, libgpp_hip.so.
+ *
+ * The library evaluates the band / G-vector loop of the BerkeleyGW GPP
+ * (General Plasmon Pole) sigma mini-app on one GPU with hand-written HIP
+ * kernels.  It replaces two seams of the reference mini-app:
+ *
+ *   1. the timed region of gppComplex.cpp:332-387 (main), which a host
+ *      driver replaces by gpp_upload() + gpp_sigma();
+ *   2. the device launchers declared at GPPComplex.h:306-310
+ *      (d_till_nvband, d_reduce_achstemp, d_noFlagOCCSolver), which take
+ *      caller-owned device pointers and accumulate into caller-zeroed device
+ *      outputs, as gppCudaVer.cpp:389-392 calls them.
+ *
+ * Conventions
+ *   - Complex values are gpp_complex {double re, im}: 16 bytes, interleaved,
+ *     layout-compatible with the reference's GPPComplex (Complex.h:15-19),
+ *     std::complex<double> and hipDoubleComplex.
+ *   - Arrays are row-major: aqsmtemp/aqsntemp [number_bands][ncouls],
+ *     I_eps_array/wtilde_array [ngpown][ncouls], vcoul [ncouls],
+ *     inv_igp_index [ngpown] (int32), wx_array [nfreq].
+ *   - igp = min(inv_igp_index[j], ncouls-1) as gppComplex.cpp:342-344; a
+ *     negative entry is rejected with GPP_ERR_BAD_INDEX (the reference reads
+ *     out of bounds there).
+ *   - Every entry point returns a gpp_status; nothing prints or exits.
+ *     gpp_last_error() gives a message for the calling thread's last error.
+ *   - Results are deterministic: the same inputs give bitwise-identical
+ *     outputs on every call (fixed-order reductions, no floating atomics).
+ *   - One context per host thread; a context is not re-entrant.
+ */
+#ifndef GPP_H
+#define GPP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPP_ABI_VERSION 1
+
+typedef enum {
+    GPP_OK = 0,
+    GPP_ERR_INVALID_ARGUMENT = 1, /* bad size, null pointer, nvband > number_bands ... */
+    GPP_ERR_BAD_INDEX = 2,        /* negative inv_igp_index entry */
+    GPP_ERR_NO_DEVICE = 3,        /* no usable gfx950 device, or HIP runtime error at init */
+    GPP_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed */
+    GPP_ERR_LAUNCH = 5,           /* kernel launch or execution error */
+    GPP_ERR_UNSUPPORTED = 6,      /* form not available for this problem */
+    GPP_ERR_NO_INPUTS = 7,        /* gpp_sigma before gpp_upload / gpp_bind_device */
+    GPP_ERR_NONFINITE = 8         /* reserved: non-finite output reported by a check */
+} gpp_status;
+
+/* One complex number, {re, im}; Complex.h:15-19. */
+typedef struct {
+    double re;
+    double im;
+} gpp_complex;
+
+/*
+ * Problem sizes.  nfreq frequencies wx_array[0..nfreq) (the reference fixes
+ * nfreq = nend - nstart = 3, gppComplex.cpp:12-13).  occ multiplies asxtemp
+ * (gppComplex.cpp:247, :363).
+ */
+typedef struct {
+    int32_t number_bands;
+    int32_t nvband;
+    int32_t ncouls;
+    int32_t ngpown;
+    int32_t nfreq;
+    double occ;
+} gpp_problem;
+
+/* Evaluation forms of the band loop. */
+typedef enum {
+    /* Direct FP64 kernel over (n1, igp, ig, iw): each update evaluates its own
+     * pole factor, complex division included. */
+    GPP_FORM_DIRECT = 0,
+    /* Direct FP64 kernel evaluating each term in the reference's exact
+     * operation order (IEEE division, no FMA contraction). */
+    GPP_FORM_DIRECT_STRICT = 1,
+    /* Band sum factored into the complex Gram product
+     * M[ig,j] = sum_n1 aqsn[n1,ig] conj(aqsm[n1,igp(j)]) on FP64 MFMA,
+     * followed by a fused FP64 pole-factor reduce. */
+    GPP_FORM_FACTORED_F64 = 2,
+    /* As GPP_FORM_FACTORED_F64 with the Gram product split into FP16 slices
+     * (Ozaki scheme) multiplied on FP16 MFMA with FP32 accumulation and
+     * recombined in FP64. */
+    GPP_FORM_FACTORED_OZAKI = 3
+} gpp_form;
+
+/* Outputs of one sigma evaluation (caller-owned host memory, overwritten).
+ * achtemp and asxtemp have nfreq entries; acht_n1_loc has number_bands
+ * entries (frequency 2 of each band, as gppComplex.cpp:381; frequency nfreq-1
+ * when nfreq < 3) and may be NULL; achstemp (reduce_achstemp,
+ * gppComplex.cpp:66-101) may be NULL. */
+typedef struct {
+    gpp_complex *achtemp;
+    gpp_complex *asxtemp;
+    gpp_complex *acht_n1_loc;
+    gpp_complex *achstemp;
+} gpp_outputs;
+
+typedef struct gpp_ctx gpp_ctx;
+
+/* ---- library / context ---------------------------------------------------- */
+
+/* Message for the calling thread's most recent failed call ("" if none). */
+const char *gpp_last_error(void);
+
+/* ABI version compiled into the library (GPP_ABI_VERSION). */
+int gpp_abi_version(void);
+
+/* Creates a context on HIP device `device`: a default-priority stream and
+ * timing events.  No input buffers are allocated until gpp_upload. */
+gpp_status gpp_ctx_create(gpp_ctx **out, int device);
+
+/* Releases the context and every device buffer it owns. Null is a no-op. */
+gpp_status gpp_ctx_destroy(gpp_ctx *ctx);
+
+/* ---- inputs ----------------------------------------------------------------- */
+
+/*
+ * Copies one problem's inputs from host memory into context-owned device
+ * buffers (synchronous; untimed, as gppCudaVer.cpp:368-377).  Buffers are
+ * reused across uploads of equal or smaller size.  Validates sizes and
+ * inv_igp_index.
+ */
+gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_igp_index,
+                      const double *wx_array, const double *vcoul, const gpp_complex *aqsmtemp,
+                      const gpp_complex *aqsntemp, const gpp_complex *I_eps_array,
+                      const gpp_complex *wtilde_array);
+
+/*
+ * Points the context at caller-owned DEVICE buffers with the same layout
+ * (no copy).  The caller keeps them alive and unchanged while the context
+ * uses them.  inv_igp_index is validated on the device.
+ */
+gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *d_inv_igp_index,
+                           const double *d_wx_array, const double *d_vcoul,
+                           const gpp_complex *d_aqsmtemp, const gpp_complex *d_aqsntemp,
+                           const gpp_complex *d_I_eps_array, const gpp_complex *d_wtilde_array);
+
+/*
+ * Fills host arrays with the reference's deterministic inputs,
+ * gppComplex.cpp:296-329: aqsmtemp = aqsntemp = (0.5+0.5i)(n1+ig),
+ * I_eps = wtilde = (0.5+0.5i)(j+ig), inv_igp_index[j] = (j+1)*ncouls/ngpown
+ * (igp_zero_based != 0 gives j*ncouls/ngpown), vcoul[ig] = ig,
+ * wx[iw] = e_lk - e_n1kq + dw*(iw-1) clamped to >= 1e-6 with e_lk=10,
+ * e_n1kq=6, dw=1.  Any output pointer may be NULL to skip that array.
+ */
+gpp_status gpp_fill_deterministic(const gpp_problem *prob, int igp_zero_based,
+                                  int32_t *inv_igp_index, double *wx_array, double *vcoul,
+                                  gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                                  gpp_complex *I_eps_array, gpp_complex *wtilde_array);
+
+/*
+ * Fills host arrays with seeded pseudo-random inputs: aqsm/aqsn re,im ~
+ * U(-1,1); I_eps re,im ~ U(-0.5,0.5); wtilde re ~ U(0.5,5), im ~
+ * U(-0.05,0.05); vcoul ~ U(0.1,10); inv_igp_index = sorted distinct random
+ * G-vector indices.  wx_array is filled as wx_center + dw*(iw - (nfreq-1)/2)
+ * (wx_center=4, dw=1 gives the reference's 3,4,5).  Element values are a
+ * counter-based hash of (seed, array, index), so any sub-range can be
+ * regenerated independently (gpp_fill_seeded_device produces the same bits).
+ */
+gpp_status gpp_fill_seeded(const gpp_problem *prob, uint64_t seed, double wx_center, double dw,
+                           int32_t *inv_igp_index, double *wx_array, double *vcoul,
+                           gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                           gpp_complex *I_eps_array, gpp_complex *wtilde_array);
+
+/*
+ * Generates the seeded inputs of gpp_fill_seeded directly into the
+ * context's own device buffers (no host copy of the large arrays; the small
+ * arrays are generated on the host and copied).  Leaves the context ready
+ * for gpp_sigma.
+ */
+gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t seed,
+                               double wx_center, double dw);
+
+/* ---- evaluation ------------------------------------------------------------- */
+
+/*
+ * Evaluates achtemp/asxtemp (and the optional outputs) for the inputs last
+ * uploaded or bound, with the given form.  Synchronous.  The GPU time of the
+ * call is available from gpp_ctx_last_timing.
+ */
+gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out);
+
+/*
+ * GPU time of the last gpp_sigma in milliseconds, measured with HIP events
+ * on the context's stream: total, and per stage (stage_ms[i] for i <
+ * *nstages, at most max_stages written; stage names via
+ * gpp_ctx_stage_name).  Any pointer may be NULL.
+ */
+gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *stage_ms,
+                               int max_stages, int *nstages);
+
+/* Name of stage i of the last gpp_sigma ("" when out of range). */
+const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i);
+
+/* ---- reference-compatible device launchers (GPPComplex.h:306-310) ---------- */
+/*
+ * Same parameter lists and meaning as the reference's CUDA launchers, with
+ * GPPComplex* replaced by the layout-identical gpp_complex*.  All pointers
+ * are DEVICE pointers owned by the caller; nfreq is fixed at 3 (nstart=0,
+ * nend=3).  Results ACCUMULATE (+=) into the caller-zeroed device outputs,
+ * as gppCudaVer.cpp:296-298,376-377 expects.  Work is enqueued on the null
+ * stream; the caller synchronises (gppCudaVer.cpp:395).  Unlike the
+ * reference these return a status, validate inv_igp_index, and are
+ * race-free and deterministic; asxtemp is accumulated (the reference never
+ * zeroes or returns it, GPPComplex.cu:239, gppCudaVer.cpp:316).
+ */
+
+/* Occupied bands n1 < nvband: achtemp_re/_im[3], asxtemp[3], acht_n1_loc[nvband].
+ * Replaces d_till_nvband, GPPComplex.h:306 / GPPComplex.cu:406-418. */
+gpp_status d_till_nvband(int nvband, int ngpown, int ncouls, int *inv_igp_index, double *wx_array,
+                         gpp_complex *wtilde_array, gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                         gpp_complex *I_eps_array, gpp_complex *asxtemp, double *vcoul,
+                         gpp_complex *acht_n1_loc, double *achtemp_re, double *achtemp_im,
+                         const double occ);
+
+/* achstemp over all bands (reduce_achstemp, gppComplex.cpp:66-101).
+ * Declared but never defined by the reference, GPPComplex.h:308. */
+gpp_status d_reduce_achstemp(int number_bands, int *inv_igp_index, int ncouls,
+                             gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                             gpp_complex *I_eps_array, double *achstemp_re, double *achstemp_im,
+                             int ngpown, double *vcoul);
+
+/* Unoccupied bands nvband <= n1 < number_bands: achtemp_re/_im[3],
+ * acht_n1_loc[number_bands].  Replaces d_noFlagOCCSolver, GPPComplex.h:310 /
+ * GPPComplex.cu:422-433. */
+gpp_status d_noFlagOCCSolver(int number_bands, int nvband, int ngpown, int ncouls,
+                             int *inv_igp_index, double *wx_array, gpp_complex *wtilde_array,
+                             gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                             gpp_complex *I_eps_array, double *vcoul, double *achtemp_re,
+                             double *achtemp_im, gpp_complex *acht_n1_loc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPP_H */

@@ -1,0 +1,90 @@
+"""Writes the reference-produced fixtures under tests/golden/.
+
+Runs in the build container only, where the reference sources exist: it
+loads oracle/_ref/libgppref.so -- the reference's own gppComplex.cpp compiled
+unmodified (main renamed) next to oracle/ref_harness.cpp by oracle/Makefile --
+feeds it the inputs below and stores inputs and outputs:
+
+  ref_sigma_vectors.npz  small band-loop cases (deterministic, seeded,
+                         near-pole, several nfreq / nvband / index clamps)
+  ref_pole_vectors.npz   ssxt_scht_solver's sch/ssx for a grid of
+                         (wx, wtilde, I_eps) covering all three branches
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import gpp_inputs  # noqa: E402
+import gpp_oracle  # noqa: E402
+
+KEYS = ("inv_igp_index", "wx_array", "vcoul", "aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array")
+
+
+def cases():
+    yield "det_6_2_48_8", gpp_inputs.det_inputs(6, 2, 48, npg=8), 2
+    yield "seeded_s1", gpp_inputs.seeded_inputs(5, 48, 6, seed=1), 2
+    yield "seeded_s2_nv0", gpp_inputs.seeded_inputs(4, 50, 7, seed=2), 0
+    yield "seeded_s3_nvall", gpp_inputs.seeded_inputs(3, 40, 5, seed=3), 3
+    yield "near_pole_s4", gpp_inputs.near_pole_inputs(5, 64, 6, seed=4), 3
+    yield "near_pole_s5", gpp_inputs.near_pole_inputs(4, 70, 5, seed=5), 4
+    yield "nfreq1_s6", gpp_inputs.seeded_inputs(3, 33, 4, nfreq=1, seed=6), 1
+    yield "nfreq5_s7", gpp_inputs.seeded_inputs(3, 41, 4, nfreq=5, seed=7, dw=0.25), 2
+    a = gpp_inputs.seeded_inputs(3, 30, 4, seed=8, sorted_index=False)
+    a["inv_igp_index"] = np.array([29, 30, 1000, 3], np.int32)  # clamped to ncouls-1
+    yield "clamp_unsorted_s8", a, 1
+
+
+def pole_grid():
+    rng = np.random.default_rng(2024)
+    wx, wt, ie = [], [], []
+    for w in (399.4, 4.0, 3.0, 0.1, -1.0):
+        for _ in range(40):
+            wx.append(w)
+            k = rng.integers(0, 5)
+            if k == 0:
+                wt.append(complex(rng.uniform(0.5, 5), rng.uniform(-0.05, 0.05)))
+            elif k == 1:
+                wt.append(complex(w + rng.uniform(-0.45, 0.45), rng.uniform(-0.01, 0.01)))
+            elif k == 2:
+                wt.append(complex(w + rng.uniform(-1e-4, 1e-4), rng.uniform(-1e-5, 1e-5)))
+            elif k == 3:
+                wt.append(complex(rng.uniform(-1e-5, 1e-5), rng.uniform(-1e-5, 1e-5)))
+            else:
+                wt.append(complex(w + 0.6 + rng.uniform(0, 0.2), rng.uniform(-0.01, 0.01)) if abs(w) > 100
+                          else complex(rng.uniform(2, 4), 0.0))
+            ie.append(complex(rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)))
+    return np.array(wx), np.array(wt), np.array(ie)
+
+
+def main():
+    if not gpp_oracle.ref_available():
+        sys.exit("oracle/_ref/libgppref.so missing: run make -C oracle ref (needs the reference sources)")
+    out = {}
+    for name, a, nv in cases():
+        r = gpp_oracle.sigma(a, nvband=nv, use_ref=True)
+        for k in KEYS:
+            out[f"{name}__{k}"] = np.ascontiguousarray(a[k])
+        out[f"{name}__nvband"] = np.array(nv)
+        for k in ("achtemp", "asxtemp", "acht_n1_loc"):
+            out[f"{name}__out_{k}"] = r[k]
+        out[f"{name}__out_achstemp"] = np.array(r["achstemp"])
+    np.savez_compressed(os.path.join(HERE, "ref_sigma_vectors.npz"), **out)
+    wx, wt, ie = pole_grid()
+    sch = np.zeros(len(wx), np.complex128)
+    ssx = np.zeros(len(wx), np.complex128)
+    for i in range(len(wx)):
+        _, sch[i], ssx[i] = gpp_oracle.pole_terms(float(wx[i]), complex(wt[i]), complex(ie[i]), use_ref=True)
+    np.savez_compressed(os.path.join(HERE, "ref_pole_vectors.npz"), wx=wx, wtilde=wt, ieps=ie, sch=sch, ssx=ssx)
+    print("wrote", len(list(cases())), "sigma cases and", len(wx), "pole vectors")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""Test inputs built independently of the library: the reference's
+deterministic initializer restated in numpy (gppComplex.cpp:296-329),
+numpy-seeded inputs with the benchmark distributions, and crafted
+near-pole inputs that drive every occupied-band branch of
+ssxt_scht_solver (gppComplex.cpp:35-56)."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def det_inputs(nb, nv, ncouls, npg=None, ngpown=None, nfreq=3, igp_zero=False):
+    """gppComplex.cpp:296-329; ngpown = ncouls // nodes_per_group (:238)."""
+    if ngpown is None:
+        ngpown = ncouls // npg
+    i = np.arange(nb, dtype=np.float64)[:, None]
+    g = np.arange(ncouls, dtype=np.float64)[None, :]
+    aq = 0.5 * (i + g) * (1 + 1j)
+    k = np.arange(ngpown, dtype=np.float64)[:, None]
+    eps = 0.5 * (k + g) * (1 + 1j)
+    jj = np.arange(ngpown, dtype=np.int64)
+    idx = ((jj if igp_zero else jj + 1) * ncouls // ngpown).astype(np.int32)
+    wx = np.array([10.0 - 6.0 + 1.0 * ((iw + 1) - 2) for iw in range(nfreq)])
+    wx = np.where(wx < 1e-6, 1e-6, wx)
+    return {
+        "inv_igp_index": idx, "wx_array": wx, "vcoul": np.arange(ncouls, dtype=np.float64),
+        "aqsmtemp": aq.copy(), "aqsntemp": aq.copy(), "I_eps_array": eps.copy(), "wtilde_array": eps.copy(),
+    }
+
+
+def seeded_inputs(nb, ncouls, ngpown, nfreq=3, seed=0, wx_center=4.0, dw=1.0, sorted_index=True):
+    """The benchmark value distributions, drawn with numpy's PCG64."""
+    rng = np.random.default_rng(seed)
+
+    def cu(shape, lo, hi, ilo, ihi):
+        return rng.uniform(lo, hi, shape) + 1j * rng.uniform(ilo, ihi, shape)
+
+    idx = rng.choice(ncouls, size=ngpown, replace=False)
+    if sorted_index:
+        idx = np.sort(idx)
+    wx = wx_center + dw * (np.arange(nfreq) - 0.5 * (nfreq - 1))
+    return {
+        "inv_igp_index": idx.astype(np.int32),
+        "wx_array": np.maximum(wx, 1e-6),
+        "vcoul": rng.uniform(0.1, 10.0, ncouls),
+        "aqsmtemp": cu((nb, ncouls), -1, 1, -1, 1),
+        "aqsntemp": cu((nb, ncouls), -1, 1, -1, 1),
+        "I_eps_array": cu((ngpown, ncouls), -0.5, 0.5, -0.5, 0.5),
+        "wtilde_array": cu((ngpown, ncouls), 0.5, 5.0, -0.05, 0.05),
+    }
+
+
+def near_pole_inputs(nb, ncouls, ngpown, seed=0):
+    """Seeded inputs with wx = (399.4, 4.0, 0.1) and wtilde placed on purpose
+    relative to those poles so that the occupied-band solver takes every
+    branch of gppComplex.cpp:35-56:
+      kind 0  default seeded wtilde                     -> branch 1
+      kind 1  4 + U(-0.45,0.45) + i U(-0.01,0.01)       -> |wdiff|^2 <= 0.25 at wx=4: branch 2
+      kind 2  4 + U(-1e-4,1e-4) + i U(-1e-5,1e-5)       -> extreme near-pole factors at wx=4
+      kind 3  U(-1e-5,1e-5) + i U(-1e-5,1e-5)           -> |delw|^2 <= 1e-6 at wx=0.1: branch 3
+      kind 4  400 + i U(-0.01,0.01)                     -> |delw|^2 >= 2.5e5 at wx=399.4: branch 2
+    """
+    a = seeded_inputs(nb, ncouls, ngpown, 3, seed)
+    rng = np.random.default_rng(seed + 1000)
+    shape = a["wtilde_array"].shape
+    kind = rng.integers(0, 5, shape)
+    wt = a["wtilde_array"]
+    u = lambda lo, hi: rng.uniform(lo, hi, shape)
+    wt = np.where(kind == 1, 4.0 + u(-0.45, 0.45) + 1j * u(-0.01, 0.01), wt)
+    wt = np.where(kind == 2, 4.0 + u(-1e-4, 1e-4) + 1j * u(-1e-5, 1e-5), wt)
+    wt = np.where(kind == 3, u(-1e-5, 1e-5) + 1j * u(-1e-5, 1e-5), wt)
+    wt = np.where(kind == 4, 400.0 + 1j * u(-0.01, 0.01), wt)
+    a["wtilde_array"] = wt
+    a["wx_array"] = np.array([399.4, 4.0, 0.1])
+    return a

@@ -1,0 +1,120 @@
+"""The C ABI without a GPU: the library loads, exports every symbol
+include/gpp.h declares, validates arguments, and its host-side input
+generators reproduce the reference's initializer (gppComplex.cpp:296-329)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpp_inputs
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "gpp.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                                 text, flags=re.M)))
+
+
+def test_exports_every_declared_symbol(gpplib):
+    import pygpp
+    funcs = header_functions()
+    assert len(funcs) >= 15
+    assert set(funcs) == set(pygpp.EXPORTED_SYMBOLS)
+    for f in funcs:
+        assert hasattr(gpplib, f), f
+    out = subprocess.run(["nm", "-D", "--defined-only", pygpp.LIB_PATH], capture_output=True, text=True).stdout
+    for f in funcs:
+        assert re.search(rf"\sT\s{f}$", out, flags=re.M), f"{f} not exported with C linkage"
+
+
+def test_abi_version(gpplib):
+    assert gpplib.gpp_abi_version() == 1
+
+
+def test_fill_deterministic_matches_reference_init(gpplib):
+    import pygpp
+    for nb, nv, ncouls, npg in [(7, 2, 50, 5), (512, 2, 800, 20)]:
+        p = pygpp.Problem(nb, nv, ncouls, ncouls // npg)
+        got = pygpp.fill_deterministic(p)
+        want = gpp_inputs.det_inputs(nb, nv, ncouls, npg=npg)
+        for k in want:
+            np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    p = pygpp.Problem(3, 1, 10, 3)
+    got = pygpp.fill_deterministic(p, igp_zero_based=True)
+    np.testing.assert_array_equal(got["inv_igp_index"], [0, 3, 6])
+
+
+def test_fill_seeded_properties(gpplib):
+    import pygpp
+    p = pygpp.Problem(6, 2, 300, 40, nfreq=16)
+    a = pygpp.fill_seeded(p, seed=7, wx_center=4.0, dw=0.25)
+    b = pygpp.fill_seeded(p, seed=7, wx_center=4.0, dw=0.25)
+    c = pygpp.fill_seeded(p, seed=8, wx_center=4.0, dw=0.25)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    assert not np.array_equal(a["aqsmtemp"], c["aqsmtemp"])
+    idx = a["inv_igp_index"]
+    assert np.all(np.diff(idx) > 0) and idx[0] >= 0 and idx[-1] < 300 and len(idx) == 40
+    for k, lo, hi, ilo, ihi in [("aqsmtemp", -1, 1, -1, 1), ("aqsntemp", -1, 1, -1, 1),
+                                ("I_eps_array", -0.5, 0.5, -0.5, 0.5), ("wtilde_array", 0.5, 5, -0.05, 0.05)]:
+        z = a[k]
+        assert z.real.min() >= lo and z.real.max() < hi and z.imag.min() >= ilo and z.imag.max() < ihi, k
+    assert a["vcoul"].min() >= 0.1 and a["vcoul"].max() < 10
+    np.testing.assert_allclose(a["wx_array"], 4.0 + 0.25 * (np.arange(16) - 7.5))
+    # the reference's frequency grid for nfreq=3, dw=1
+    np.testing.assert_array_equal(pygpp.fill_seeded(pygpp.Problem(1, 0, 4, 2), 1)["wx_array"], [3.0, 4.0, 5.0])
+    # sub-arrays regenerate identically whatever the problem size (counter-based)
+    q = pygpp.fill_seeded(pygpp.Problem(6, 2, 300, 40, nfreq=3), seed=7)
+    np.testing.assert_array_equal(q["aqsmtemp"], a["aqsmtemp"])
+
+
+def test_invalid_arguments_rejected(gpplib):
+    import pygpp
+    for bad in [pygpp.Problem(0, 0, 10, 2), pygpp.Problem(4, 5, 10, 2), pygpp.Problem(4, 1, 0, 2),
+                pygpp.Problem(4, 1, 10, 0), pygpp.Problem(4, 1, 10, 2, nfreq=0)]:
+        with pytest.raises(pygpp.GppError) as e:
+            pygpp.fill_deterministic(bad)
+        assert e.value.status == pygpp.GPP_ERR_INVALID_ARGUMENT
+    with pytest.raises(pygpp.GppError) as e:
+        pygpp.fill_seeded(pygpp.Problem(2, 1, 10, 11), 1)  # more columns than G-vectors
+    assert e.value.status == pygpp.GPP_ERR_INVALID_ARGUMENT
+    # null context / null outputs
+    assert gpplib.gpp_sigma(None, 0, None) == pygpp.GPP_ERR_INVALID_ARGUMENT
+    assert gpplib.gpp_ctx_destroy(None) == pygpp.GPP_OK
+    assert b"NULL" in gpplib.gpp_last_error() or gpplib.gpp_last_error()
+    # reference-compatible launchers validate sizes before touching the device
+    assert gpplib.d_noFlagOCCSolver(4, 5, 2, 10, *([None] * 10)) == pygpp.GPP_ERR_INVALID_ARGUMENT
+    assert gpplib.d_till_nvband(-1, 2, 10, *([None] * 11), 1.0) == pygpp.GPP_ERR_INVALID_ARGUMENT
+    assert gpplib.d_reduce_achstemp(0, None, 10, None, None, None, None, None, 2, None) == \
+        pygpp.GPP_ERR_INVALID_ARGUMENT
+
+
+def _gpu_visible():
+    return os.path.exists("/dev/kfd") and any(
+        os.path.exists(f"/dev/dri/renderD{128 + i}") for i in range(16))
+
+
+def test_no_device_reported_without_gpu(gpplib):
+    import pygpp
+    if _gpu_visible():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(pygpp.GppError) as e:
+        pygpp.Context(0)
+    assert e.value.status == pygpp.GPP_ERR_NO_DEVICE
+
+
+def test_driver_usage_error_exit_status(gpplib):
+    drv = os.path.join(REPO, "doe-csfg-hackathon_amd", "bin", "gpp_driver")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    r = subprocess.run([drv, "1", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "correct form of input" in r.stdout
+    r = subprocess.run([drv, "4", "1", "10", "2", "--form", "bogus"], capture_output=True, text=True)
+    assert r.returncode == 2

@@ -1,0 +1,216 @@
+"""Parity of the HIP direct form (libgpp_hip.so through its C ABI) with the
+oracle and with the reference's own outputs.  Tolerance: 1e-12 relative per
+component (the direct kernel's bound against the reference CPU build); a
+component smaller than 1e-6 of its complex modulus is compared relative to
+the modulus."""
+import numpy as np
+import pytest
+
+import gpp_inputs
+import hiprt
+from test_oracle import README_GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+FORMS_DIRECT = ["direct", "direct-strict"]
+
+
+def comp_rel_err(got, ref):
+    got = np.atleast_1d(np.asarray(got, np.complex128))
+    ref = np.atleast_1d(np.asarray(ref, np.complex128))
+    worst = 0.0
+    for g, r in zip(got, ref):
+        m = abs(r)
+        for gc, rc in ((g.real, r.real), (g.imag, r.imag)):
+            den = max(abs(rc), 1e-6 * m, 1e-300)
+            worst = max(worst, abs(gc - rc) / den)
+    return worst
+
+
+def assert_close(out, ref, tol=TOL, keys=("achtemp", "asxtemp", "acht_n1_loc", "achstemp")):
+    for k in keys:
+        e = comp_rel_err(out[k], ref[k])
+        assert e <= tol, f"{k}: rel err {e:.3e} > {tol:.1e}\n got {out[k]}\n ref {ref[k]}"
+
+
+def problem_of(arrs, nv):
+    import pygpp
+    nb, ncouls = arrs["aqsmtemp"].shape
+    return pygpp.Problem(nb, nv, ncouls, arrs["I_eps_array"].shape[0], len(arrs["wx_array"]))
+
+
+def run_gpu(ctx, arrs, nv, form):
+    ctx.upload(problem_of(arrs, nv), arrs)
+    return ctx.sigma(form)
+
+
+@pytest.mark.parametrize("ncouls", [800, 1600, 3200])
+@pytest.mark.parametrize("form", FORMS_DIRECT)
+def test_readme_golden(ctx, oracle, ncouls, form):
+    arrs = gpp_inputs.det_inputs(512, 2, ncouls, npg=20)
+    out = run_gpu(ctx, arrs, 2, form)
+    gold = np.array([complex(r, i) for r, i in README_GOLDEN[ncouls]])
+    assert comp_rel_err(out["achtemp"], gold) <= TOL
+    assert_close(out, oracle.sigma(arrs, nvband=2, threads=8))
+
+
+SHAPES = [
+    # nb, nv, ncouls, ngpown, nfreq
+    (16, 2, 1000, 37, 3),
+    (13, 3, 777, 21, 3),     # ragged everywhere
+    (9, 0, 130, 9, 3),       # no occupied bands
+    (5, 5, 200, 6, 3),       # only occupied bands
+    (11, 2, 64, 4, 1),
+    (11, 2, 65, 5, 2),
+    (10, 1, 300, 10, 4),
+    (10, 2, 300, 10, 5),
+    (6, 2, 256, 8, 16),
+    (1, 0, 1, 1, 3),         # smallest problem
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("form", FORMS_DIRECT)
+def test_seeded_vs_oracle(ctx, oracle, shape, form):
+    nb, nv, ncouls, ngpown, nfreq = shape
+    arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=sum(shape), dw=0.25 if nfreq > 5 else 1.0)
+    assert_close(run_gpu(ctx, arrs, nv, form), oracle.sigma(arrs, nvband=nv, threads=8))
+
+
+@pytest.mark.parametrize("form", FORMS_DIRECT)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_near_pole_vs_oracle(ctx, oracle, form, seed):
+    arrs = gpp_inputs.near_pole_inputs(12, 1500, 40, seed=seed)
+    assert_close(run_gpu(ctx, arrs, 4, form), oracle.sigma(arrs, nvband=4, threads=8))
+
+
+@pytest.mark.parametrize("form", FORMS_DIRECT)
+def test_reference_fixtures(ctx, form):
+    """Inputs and outputs written by the compiled reference (tests/golden)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_sigma_vectors.npz"))
+    names = sorted({k.split("__")[0] for k in z.files})
+    for name in names:
+        a = {k: z[f"{name}__{k}"] for k in ("inv_igp_index", "wx_array", "vcoul", "aqsmtemp", "aqsntemp",
+                                            "I_eps_array", "wtilde_array")}
+        nv = int(z[f"{name}__nvband"])
+        out = run_gpu(ctx, a, nv, form)
+        ref = {k: z[f"{name}__out_{k}"] for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")}
+        assert_close(out, ref)
+
+
+def test_bitwise_deterministic(ctx):
+    arrs = gpp_inputs.near_pole_inputs(20, 3000, 50, seed=9)
+    ctx.upload(problem_of(arrs, 3), arrs)
+    a = ctx.sigma("direct")
+    for _ in range(3):
+        b = ctx.sigma("direct")
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_device_generator_matches_host(ctx):
+    import pygpp
+    p = pygpp.Problem(12, 2, 2000, 100, nfreq=3)
+    host = pygpp.fill_seeded(p, seed=42)
+    ctx.upload(p, host)
+    a = ctx.sigma("direct")
+    ctx.generate_seeded(p, seed=42)
+    b = ctx.sigma("direct")
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_negative_index_rejected(ctx):
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(4, 50, 5, seed=1)
+    arrs["inv_igp_index"][2] = -1
+    with pytest.raises(pygpp.GppError) as e:
+        ctx.upload(problem_of(arrs, 1), arrs)
+    assert e.value.status == pygpp.GPP_ERR_BAD_INDEX
+    dev = {k: hiprt.DeviceArray(v) for k, v in arrs.items()}
+    with pytest.raises(pygpp.GppError) as e:
+        ctx.bind_device(problem_of(arrs, 1), {k: d.ptr for k, d in dev.items()})
+    assert e.value.status == pygpp.GPP_ERR_BAD_INDEX
+
+
+def test_nan_at_pole_propagates(ctx, oracle):
+    arrs = gpp_inputs.seeded_inputs(4, 100, 6, seed=3)
+    arrs["wtilde_array"][2, 17] = arrs["wx_array"][1]  # exactly on the pole
+    for form in FORMS_DIRECT:
+        out = run_gpu(ctx, arrs, 1, form)
+        ref = oracle.sigma(arrs, nvband=1)
+        assert np.isnan(out["achtemp"][1].real) and np.isnan(ref["achtemp"][1].real)
+        assert_close({"achtemp": out["achtemp"][[0, 2]]}, {"achtemp": ref["achtemp"][[0, 2]]}, keys=("achtemp",))
+
+
+def test_bind_device_matches_upload(ctx):
+    arrs = gpp_inputs.seeded_inputs(10, 900, 30, seed=5)
+    p = problem_of(arrs, 2)
+    ctx.upload(p, arrs)
+    a = ctx.sigma("direct")
+    dev = {k: hiprt.DeviceArray(v) for k, v in arrs.items()}
+    ctx.bind_device(p, {k: d.ptr for k, d in dev.items()})
+    b = ctx.sigma("direct")
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_reference_compatible_launchers(ctx, gpplib):
+    """d_till_nvband / d_noFlagOCCSolver / d_reduce_achstemp with device
+    pointers, as gppCudaVer.cpp:385-399 drives them: results accumulate into
+    caller-zeroed device outputs."""
+    arrs = gpp_inputs.near_pole_inputs(9, 700, 23, seed=6)
+    nb, nv, ncouls, ngpown = 9, 3, 700, 23
+    ctx.upload(problem_of(arrs, nv), arrs)
+    want = ctx.sigma("direct")
+    d = {k: hiprt.DeviceArray(v) for k, v in arrs.items()}
+    ach_re = hiprt.DeviceArray(np.zeros(3))
+    ach_im = hiprt.DeviceArray(np.zeros(3))
+    asx = hiprt.DeviceArray(np.zeros(3, np.complex128))
+    n1loc = hiprt.DeviceArray(np.zeros(nb, np.complex128))
+    achs_re = hiprt.DeviceArray(np.zeros(1))
+    achs_im = hiprt.DeviceArray(np.zeros(1))
+    for rep in (1, 2):
+        assert gpplib.d_reduce_achstemp(nb, d["inv_igp_index"].ptr, ncouls, d["aqsmtemp"].ptr, d["aqsntemp"].ptr,
+                                        d["I_eps_array"].ptr, achs_re.ptr, achs_im.ptr, ngpown, d["vcoul"].ptr) == 0
+        assert gpplib.d_till_nvband(nv, ngpown, ncouls, d["inv_igp_index"].ptr, d["wx_array"].ptr,
+                                    d["wtilde_array"].ptr, d["aqsmtemp"].ptr, d["aqsntemp"].ptr,
+                                    d["I_eps_array"].ptr, asx.ptr, d["vcoul"].ptr, n1loc.ptr, ach_re.ptr,
+                                    ach_im.ptr, 1.0) == 0
+        assert gpplib.d_noFlagOCCSolver(nb, nv, ngpown, ncouls, d["inv_igp_index"].ptr, d["wx_array"].ptr,
+                                        d["wtilde_array"].ptr, d["aqsmtemp"].ptr, d["aqsntemp"].ptr,
+                                        d["I_eps_array"].ptr, d["vcoul"].ptr, ach_re.ptr, ach_im.ptr,
+                                        n1loc.ptr) == 0
+        got = {"achtemp": ach_re.get() + 1j * ach_im.get(), "asxtemp": asx.get(), "acht_n1_loc": n1loc.get(),
+               "achstemp": achs_re.get()[0] + 1j * achs_im.get()[0]}
+        ref = {k: rep * np.asarray(want[k]) for k in got}
+        assert_close(got, ref, tol=1e-13)
+
+
+def test_full_c1_shape_band_subset(ctx, oracle):
+    """The benchmark shape (ncouls=32768, ngpown=1638, nfreq=3) on 8 of its
+    bands, 2 occupied, against the oracle."""
+    import pygpp
+    p = pygpp.Problem(8, 2, 32768, 1638)
+    arrs = pygpp.fill_seeded(p, seed=1)
+    ctx.upload(p, arrs)
+    ref = oracle.sigma(arrs, nvband=2, threads=16)
+    for form in FORMS_DIRECT:
+        assert_close(ctx.sigma(form), ref)
+
+
+def test_full_c1_properties(ctx):
+    """Full benchmark size: strict and fast direct agree, band sums close, and
+    repeated evaluations are bitwise identical."""
+    import pygpp
+    p = pygpp.Problem(512, 2, 32768, 1638)
+    ctx.generate_seeded(p, seed=1)
+    a = ctx.sigma("direct")
+    b = ctx.sigma("direct-strict")
+    assert_close(a, b)
+    np.testing.assert_allclose(a["acht_n1_loc"].sum(), a["achtemp"][2], rtol=1e-11)
+    c = ctx.sigma("direct")
+    for k in a:
+        np.testing.assert_array_equal(a[k], c[k])
