@@ -120,7 +120,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", default="direct", choices=["direct", "direct-strict", "factored-f64", "factored-ozaki"])
+    ap.add_argument("--form", default="factored-f64",
+                    choices=["direct", "direct-strict", "factored-f64", "factored-ozaki"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -137,34 +138,59 @@ def main():
     ctx = pygpp.Context(0)
     ctx.generate_seeded(prob, SEED)
 
+    run_forms = [args.form] + [f for f in ("factored-f64", "direct", "direct-strict") if f != args.form]
     forms = {}
     results = {}
-    for form in ["direct", "direct-strict"] if args.form.startswith("direct") else [args.form, "direct"]:
+    for form in run_forms:
         out, ms, stages = time_form(ctx, form, args.steps if form == args.form else max(1, args.steps // 2),
                                     args.warmup)
         results[form] = out
         forms[form] = {"ms_per_step": ms, "updates_per_s": updates / (ms * 1e-3), "gpu_stage_ms": stages}
     head = forms[args.form]
-    # agreement between forms (the direct forms are checked against the CPU oracle by the tests)
-    base = results["direct-strict"] if "direct-strict" in results else results["direct"]
+    # Agreement with the strict direct form (reference operation order, FP64; the
+    # direct forms are themselves checked against the CPU oracle by the tests).
+    base = results["direct-strict"]
     for form, out in results.items():
-        forms[form]["max_rel_err_vs_" + ("direct-strict" if "direct-strict" in results else "direct")] = max(
-            rel_err(out[k], base[k]) for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp"))
+        keys = ("achtemp", "asxtemp", "achstemp") + (("acht_n1_loc",) if out.get("acht_n1_loc") is not None else ())
+        forms[form]["max_rel_err_vs_direct_strict"] = max(rel_err(out[k], base[k]) for k in keys)
+    if "factored-f64" in forms:
+        forms["factored-f64"]["speedup_vs_direct"] = forms["direct"]["ms_per_step"] / forms["factored-f64"]["ms_per_step"]
 
-    # roofline of the dominant kernel of the headline form
-    if args.form.startswith("direct"):
-        kname = "direct_unoccupied_strict" if args.form == "direct-strict" else "direct_unoccupied"
-        kms = head["gpu_stage_ms"][kname]
+    def direct_roofline(form):
+        kname = "direct_unoccupied_strict" if form == "direct-strict" else "direct_unoccupied"
+        kms = forms[form]["gpu_stage_ms"][kname]
         flop = (NB - NV) * NGPOWN * NCOULS * NFREQ * FLOP_UNOCC
         achieved = flop / (kms * 1e-3) / 1e12
         tr = load_traffic("k_unocc")
-        roofline = {"bound": "mfma", "pipe": "FP64 VALU (FP64 vector peak = FP64 matrix peak on MI355X)",
-                    "kernel": kname, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / PEAK_FP64_TFLOPS,
-                    "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
-                    "algorithmic_flop_per_launch": flop, "avg_launch_ms": kms}
-    else:
-        roofline = None
+        return {"bound": "mfma", "pipe": "FP64 VALU (FP64 vector peak = FP64 matrix peak on MI355X)",
+                "kernel": kname, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+                "algorithmic_flop_per_launch": flop, "avg_launch_ms": kms}
+
+    def factored_roofline(form):
+        st = forms[form]["gpu_stage_ms"]
+        gms = st["factored_gram_f64"]
+        flop = 8 * NGPOWN * NCOULS * (NB - NV)
+        achieved = flop / (gms * 1e-3) / 1e12
+        tr = load_traffic("k_gram_f64")
+        red_ms = st["factored_pole_reduce"]
+        red_bytes = 3 * 16 * NGPOWN * NCOULS  # wtilde, I_eps, Mu
+        tr2 = load_traffic("k_pole_reduce")
+        return {"bound": "mfma", "kernel": "factored_gram_f64 (v_mfma_f64_16x16x4)", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+                "algorithmic_flop_per_launch": flop, "avg_launch_ms": gms,
+                "second_kernel": {"bound": "hbm", "kernel": "factored_pole_reduce", "avg_launch_ms": red_ms,
+                                  "algorithmic_bytes": red_bytes,
+                                  "achieved": red_bytes / (red_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": red_bytes / (red_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                  "traffic": (tr2["fetch_bytes"] + tr2["write_bytes"]) if tr2 else None}}
+
+    roofline = direct_roofline(args.form) if args.form.startswith("direct") else factored_roofline(args.form)
+    for form in forms:
+        if form != args.form:
+            forms[form]["roofline"] = direct_roofline(form) if form.startswith("direct") else factored_roofline(form)
 
     line = {
         "metric": METRIC,

@@ -440,6 +440,9 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (!out || !out->achtemp || !out->asxtemp)
         return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
     if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    if ((form == GPP_FORM_FACTORED_F64 || form == GPP_FORM_FACTORED_OZAKI) && out->acht_n1_loc)
+        return fail(GPP_ERR_UNSUPPORTED, "acht_n1_loc is a per-band sum and comes from the direct forms only; "
+                                         "pass NULL with the factored forms");
     gpp_status st = set_device(ctx);
     if (st) return st;
     const gpp::DevProblem &dp = ctx->dp;

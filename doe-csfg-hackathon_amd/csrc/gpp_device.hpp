@@ -106,6 +106,43 @@ __device__ __forceinline__ cplx block_sum(cplx v, cplx *scratch)
     return r;
 }
 
+constexpr double kTo1 = 1e-6;    // to1, gppComplex.cpp:20, :241
+constexpr double kSexcut = 4.0;  // sexcut, gppComplex.cpp:21, :243
+
+// Occupied-band pole terms, ssxt_scht_solver (gppComplex.cpp:16-63), in the
+// reference's operation order with IEEE division and no contraction, so the
+// branch decisions are taken on the same values as the reference.
+// wx2 = wxt^2 and rwx4 = 1/wx2^2 (:38-40) are passed in.
+__device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rwx4, cplx wt, cplx ie,
+                                               cplx &sch, cplx &ssx)
+{
+#pragma clang fp contract(off)
+    const double limitone = 1.0 / (kTo1 * 4.0);  // :22
+    const double limittwo = 0.25;                // :23, pow(0.5,2)
+    const cplx wt2 = csquare(wt);                // flagOCC_solver :117
+    const cplx om2 = cmul(wt2, ie);              // :118
+    sch = czero();
+    ssx = czero();
+    const cplx wdiff = cdminus(wxt, wt);                         // :27
+    const double rden = 1 / cmul(wdiff, cconj(wdiff)).re;        // :30
+    const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);      // :31
+    const double delwr = cmul(delw, cconj(delw)).re;             // :32
+    const double wdiffr = cmul(wdiff, cconj(wdiff)).re;          // :33
+    if ((wdiffr > limittwo) && (delwr < limitone)) {             // :35
+        sch = cmul(delw, ie);                                    // :37
+        ssx = cscale2(om2, wx2, rwx4);                           // :38-41
+    } else if (delwr > kTo1) {                                   // :43
+        const cplx cden = cscale(cmul(wt2, cdplus(0.50, delw)), 4.00);  // :46
+        const double rd = 1.00 / cmul(cden, cconj(cden)).re;            // :48-49
+        ssx = cmul(cmul(cplx{-om2.re, -om2.im}, cconj(cden)), cscale(delw, rd));  // :50
+    }
+    // ssx cutoff, :58-59 (only reachable for wx < 0)
+    if (wxt < 0.00) {
+        const double ssxcutoff = cabs_(ie) * kSexcut;
+        if (cabs_(ssx) > ssxcutoff) ssx = czero();
+    }
+}
+
 // igp = min(inv_igp_index[j], ncouls-1), gppComplex.cpp:342-344.  Negative
 // entries are rejected before any launch; the max(.,0) only keeps a kernel
 // in bounds should one slip through.

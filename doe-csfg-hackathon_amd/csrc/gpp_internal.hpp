@@ -73,6 +73,9 @@ hipError_t run_direct(const DevProblem &p, bool strict, void *work, const DevRes
 hipError_t run_direct_part(const DevProblem &p, int which, void *work, const DevResults &res,
                            hipStream_t s);
 
+// reduce_achstemp per band into part_achs[nb] (fixed-order sums over j).
+hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, hipStream_t s);
+
 // Counts negative inv_igp_index entries on the device (result in *d_count).
 hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hipStream_t s);
 

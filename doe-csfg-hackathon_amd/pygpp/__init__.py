@@ -239,14 +239,18 @@ class Context:
         _check(load().gpp_generate_seeded(self._h, ctypes.byref(prob), seed, wx_center, dw))
         self.problem = p
 
-    def sigma(self, form: str = "direct") -> Dict[str, np.ndarray]:
+    def sigma(self, form: str = "direct", acht_n1_loc: Optional[bool] = None) -> Dict[str, np.ndarray]:
+        """Evaluates the band loop.  acht_n1_loc (per-band sums) is requested
+        by default from the direct forms only; the factored forms reject it."""
         if self.problem is None:
             raise GppError(GPP_ERR_NO_INPUTS, "no inputs")
+        if acht_n1_loc is None:
+            acht_n1_loc = form.startswith("direct")
         p = self.problem
         out = {
             "achtemp": np.zeros(p.nfreq, np.complex128),
             "asxtemp": np.zeros(p.nfreq, np.complex128),
-            "acht_n1_loc": np.zeros(p.number_bands, np.complex128),
+            "acht_n1_loc": np.zeros(p.number_bands, np.complex128) if acht_n1_loc else None,
             "achstemp": np.zeros(1, np.complex128),
         }
         o = GppOutputs(*[_ptr(out[k]) for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")])

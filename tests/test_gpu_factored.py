@@ -1,0 +1,83 @@
+"""Parity of the factored forms (band sum through the complex Gram product on
+MFMA, then the FP64 pole-factor reduce) with the FP64 oracle.  Bound: 1e-10
+relative per component (components below 1e-6 of their modulus are compared
+relative to the modulus).  The FP64-MFMA form is expected near 1e-14."""
+import numpy as np
+import pytest
+
+import gpp_inputs
+from test_gpu_parity import assert_close, comp_rel_err, problem_of, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+TOL_FACTORED = 1e-10
+KEYS = ("achtemp", "asxtemp", "achstemp")
+FACTORED = ["factored-f64"]
+
+SHAPES = [
+    (16, 2, 1000, 37, 3),
+    (13, 3, 777, 21, 3),
+    (9, 0, 130, 9, 3),
+    (5, 5, 200, 6, 3),
+    (11, 2, 65, 5, 2),
+    (10, 2, 300, 10, 5),
+    (6, 2, 256, 8, 16),
+    (130, 4, 300, 140, 3),   # K not a multiple of the band stage; several Gram tiles
+    (1, 0, 1, 1, 3),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_seeded_vs_oracle(ctx, oracle, shape, form):
+    nb, nv, ncouls, ngpown, nfreq = shape
+    arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=sum(shape) + 7,
+                                    dw=0.25 if nfreq > 5 else 1.0)
+    assert_close(run_gpu(ctx, arrs, nv, form), oracle.sigma(arrs, nvband=nv, threads=8), TOL_FACTORED, KEYS)
+
+
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_near_pole(ctx, oracle, form):
+    arrs = gpp_inputs.near_pole_inputs(40, 1500, 70, seed=3)
+    assert_close(run_gpu(ctx, arrs, 4, form), oracle.sigma(arrs, nvband=4, threads=8), TOL_FACTORED, KEYS)
+
+
+@pytest.mark.parametrize("ncouls", [800, 3200])
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_readme_golden(ctx, oracle, ncouls, form):
+    from test_oracle import README_GOLDEN
+    arrs = gpp_inputs.det_inputs(512, 2, ncouls, npg=20)
+    out = run_gpu(ctx, arrs, 2, form)
+    gold = np.array([complex(r, i) for r, i in README_GOLDEN[ncouls]])
+    assert comp_rel_err(out["achtemp"], gold) <= TOL_FACTORED
+    assert_close(out, oracle.sigma(arrs, nvband=2, threads=8), TOL_FACTORED, KEYS)
+
+
+def test_factored_rejects_acht_n1_loc(ctx):
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(4, 64, 4, seed=2)
+    ctx.upload(problem_of(arrs, 1), arrs)
+    with pytest.raises(pygpp.GppError) as e:
+        ctx.sigma("factored-f64", acht_n1_loc=True)
+    assert e.value.status == pygpp.GPP_ERR_UNSUPPORTED
+
+
+def test_factored_full_c1_band_subset(ctx, oracle):
+    import pygpp
+    p = pygpp.Problem(24, 2, 32768, 1638)
+    arrs = pygpp.fill_seeded(p, seed=3)
+    ctx.upload(p, arrs)
+    assert_close(ctx.sigma("factored-f64"), oracle.sigma(arrs, nvband=2, threads=16), TOL_FACTORED, KEYS)
+
+
+def test_factored_full_c1_vs_direct_strict(ctx):
+    """Full benchmark size: factored FP64 against the strict direct kernel."""
+    import pygpp
+    p = pygpp.Problem(512, 2, 32768, 1638)
+    ctx.generate_seeded(p, seed=1)
+    ref = ctx.sigma("direct-strict")
+    out = ctx.sigma("factored-f64")
+    assert_close(out, ref, TOL_FACTORED, KEYS)
+    again = ctx.sigma("factored-f64")
+    for k in KEYS:
+        np.testing.assert_array_equal(out[k], again[k])
