@@ -35,6 +35,7 @@ constexpr int kKS = 8;        // bands per LDS stage (two MFMA k-steps of 4)
 constexpr int kGramThreads = 512;  // 8 waves: 2 (j) x 4 (ig), wave tile 64 x 32
 constexpr int kWJ = 64, kWI = 32;  // wave tile
 constexpr int kRedThreads = 256;
+constexpr int kRedChunk = 8192;    // G-vectors per pole-reduce block
 
 __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -48,8 +49,9 @@ FactLayout fact_layout(const DevProblem &p)
     size_t off = 0;
     L.bg = off;        off += align256(sizeof(cplx) * (size_t)p.nb * p.ngpown);
     L.m = off;         off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
-    L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * p.nfreq);
-    L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * p.nfreq);
+    const size_t nsplit = (size_t)(p.ncouls + kRedChunk - 1) / kRedChunk;
+    L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
+    L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
     L.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
     L.total = off;
     return L;
@@ -173,20 +175,31 @@ struct ReduceArgs {
     const cplx *bg;      // [nb][ngpown]
     const int32_t *idx;
     const double *vcoul, *wx;
-    int nv, ncouls, ngpown, nfreq;
+    int nv, ncouls, ngpown, nfreq, nsplit;
     double occ;
-    cplx *col_ach, *col_asx;  // [ngpown][nfreq]
+    cplx *col_ach, *col_asx;  // [ngpown][nsplit][nfreq]
 };
 
-// Stage 3: one block per column j (grid.x) and frequency chunk (grid.y).
+// Stage 3: block (j, split, frequency chunk) reduces the pole terms of column j
+// over G-vectors [split*kRedChunk, +kRedChunk).  Per element and frequency:
+//   delw = wt*conj(wdiff)*rden and P = delw*ie in the reference's order
+//   (gppComplex.cpp:139-146, no contraction; rden = the refined reciprocal, equal
+//   to the IEEE quotient), so the occupied branch tests (:35, :43) see the
+//   reference's values; sch = P in branch 1 (:37) is shared with the
+//   unoccupied term, and ssx follows :38-51.
 template <int FC>
 __global__ __launch_bounds__(kRedThreads) void k_pole_reduce(ReduceArgs a)
 {
+#pragma clang fp contract(off)
     __shared__ cplx scratch[kRedThreads / kWave];
     const int j = blockIdx.x;
-    const int iw0 = blockIdx.y * FC;
+    const int split = blockIdx.y;
+    const int iw0 = blockIdx.z * FC;
     const int ncouls = a.ncouls;
     const int nv = a.nv;
+    const int ig_begin = split * kRedChunk;
+    const int ig_end = ig_begin + kRedChunk < ncouls ? ig_begin + kRedChunk : ncouls;
+    const double limitone = 1.0 / (kTo1 * 4.0);
     double wxv[FC], wx2[FC], rwx4[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
@@ -199,60 +212,68 @@ __global__ __launch_bounds__(kRedThreads) void k_pole_reduce(ReduceArgs a)
     for (int f = 0; f < FC; ++f) sach[f] = sasx[f] = czero();
 
     const size_t row = (size_t)j * ncouls;
-    for (int ig = threadIdx.x; ig < ncouls; ig += blockDim.x) {
+    for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
         const cplx wt = a.wtilde[row + ig];
         const cplx ie = a.ieps[row + ig];
         const cplx mu = a.mu[row + ig];
-        cplx mo = czero();
-        for (int n1 = 0; n1 < nv; ++n1)  // occupied bands, in band order
-            mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
-        // P * Mu with P = wt * conj(wdiff) / |wdiff|^2 * ie, re-associated as in the
-        // direct kernel: ((Mu * (wt * ie)) * conj(wdiff)) / |wdiff|^2.
-        const cplx q = cmul_fast(mu, cmul_fast(wt, ie));
+        cplx mo = czero(), om2 = czero();
+        if (nv > 0) {
+            for (int n1 = 0; n1 < nv; ++n1)  // occupied bands, in band order
+                mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
+            om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
+        }
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
-            const double wdr = wxv[f] - wt.re;
-            const double d = wdr * wdr + wt.im * wt.im;
-            const double r = rcp_refined(d);
-            const double tr = q.re * wdr - q.im * wt.im;
-            const double ti = q.im * wdr + q.re * wt.im;
-            sach[f].re += tr * r;
-            sach[f].im += ti * r;
+            const cplx wdiff = cdminus(wxv[f], wt);
+            const double wdiffr = cmul(wdiff, cconj(wdiff)).re;
+            const double rden = rcp_refined(wdiffr);
+            const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);
+            const cplx p = cmul(delw, ie);
+            cplx m = mu;
             if (nv > 0) {
-                cplx sch, ssx;
-                occ_pole_terms(wxv[f], wx2[f], rwx4[f], wt, ie, sch, ssx);
-                const cplx c1 = cmul_fast(mo, sch);
-                const cplx c2 = cmul_fast(mo, ssx);
-                sach[f] = cadd(sach[f], c1);
-                sasx[f] = cadd(sasx[f], c2);
+                const double delwr = cmul(delw, cconj(delw)).re;
+                cplx ssx = czero();
+                if ((wdiffr > 0.25) && (delwr < limitone)) {
+                    m = cadd(mu, mo);                  // sch = P for the occupied bands
+                    ssx = cscale2(om2, wx2[f], rwx4[f]);
+                } else if (delwr > kTo1) {
+                    const cplx cden = cscale(cmul(csquare(wt), cdplus(0.50, delw)), 4.00);
+                    const double rd = rcp_refined(cmul(cden, cconj(cden)).re);
+                    ssx = cmul(cmul(cplx{-om2.re, -om2.im}, cconj(cden)), cscale(delw, rd));
+                }
+                if (wxv[f] < 0.00 && cabs_(ssx) > cabs_(ie) * kSexcut) ssx = czero();
+                const cplx x = cmul_fast(ssx, mo);
+                sasx[f] = cadd(sasx[f], x);
             }
+            const cplx t = cmul_fast(p, m);
+            sach[f] = cadd(sach[f], t);
         }
     }
-    const int igp = igp_of(a.idx, j, ncouls);
-    const double vc = a.vcoul[igp];
+    const double vc = a.vcoul[igp_of(a.idx, j, ncouls)];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
         const cplx s = block_sum(sach[f], scratch);
         const cplx x = block_sum(sasx[f], scratch);
         if (threadIdx.x == 0 && iw0 + f < a.nfreq) {
             // (0.5 * sum) * vcoul and sum * (occ * vcoul), gppComplex.cpp:361-363, :379
-            a.col_ach[(size_t)j * a.nfreq + iw0 + f] = cscale(cscale(s, 0.5), vc);
-            a.col_asx[(size_t)j * a.nfreq + iw0 + f] = cscale(x, a.occ * vc);
+            const size_t o = ((size_t)j * a.nsplit + split) * a.nfreq + iw0 + f;
+            a.col_ach[o] = cscale(cscale(s, 0.5), vc);
+            a.col_asx[o] = cscale(x, a.occ * vc);
         }
     }
 }
 
 // Stage 4: achtemp/asxtemp over columns (fixed order), achstemp over bands.
 __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach, const cplx *col_asx,
-                                                            const cplx *part_achs, int ngpown, int nfreq, int nb,
+                                                            const cplx *part_achs, int ncols, int nfreq, int nb,
                                                             int nv, cplx *out)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     for (int iw = 0; iw < nfreq; ++iw) {
         cplx s = czero(), x = czero();
-        for (int j = threadIdx.x; j < ngpown; j += blockDim.x) {
-            s = cadd(s, col_ach[(size_t)j * nfreq + iw]);
-            x = cadd(x, col_asx[(size_t)j * nfreq + iw]);
+        for (int c = threadIdx.x; c < ncols; c += blockDim.x) {  // (column, split) pairs in order
+            s = cadd(s, col_ach[(size_t)c * nfreq + iw]);
+            x = cadd(x, col_asx[(size_t)c * nfreq + iw]);
         }
         s = block_sum(s, scratch);
         x = block_sum(x, scratch);
@@ -272,7 +293,7 @@ __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach,
 template <int FC>
 void launch_reduce_fc(const ReduceArgs &a, hipStream_t s)
 {
-    dim3 grid(a.ngpown, (a.nfreq + FC - 1) / FC);
+    dim3 grid(a.ngpown, a.nsplit, (a.nfreq + FC - 1) / FC);
     hipLaunchKernelGGL((k_pole_reduce<FC>), grid, dim3(kRedThreads), 0, s, a);
 }
 
@@ -323,6 +344,7 @@ hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevRe
     ra.mu = mu; ra.ieps = p.ieps; ra.wtilde = p.wtilde; ra.aqsn = p.aqsn; ra.bg = bg; ra.idx = p.idx;
     ra.vcoul = p.vcoul; ra.wx = p.wx; ra.nv = p.nv; ra.ncouls = p.ncouls; ra.ngpown = p.ngpown;
     ra.nfreq = p.nfreq; ra.occ = p.occ; ra.col_ach = col_ach; ra.col_asx = col_asx;
+    ra.nsplit = (p.ncouls + kRedChunk - 1) / kRedChunk;
     switch (p.nfreq) {
     case 1: launch_reduce_fc<1>(ra, s); break;
     case 2: launch_reduce_fc<2>(ra, s); break;
@@ -332,8 +354,8 @@ hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevRe
     }
     GPP_TRY(hipGetLastError());
     if (timer) timer->mark(s, "factored_pole_reduce");
-    hipLaunchKernelGGL(k_fact_final, dim3(1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs, p.ngpown,
-                       p.nfreq, p.nb, p.nv, res.base);
+    hipLaunchKernelGGL(k_fact_final, dim3(1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs,
+                       p.ngpown * ra.nsplit, p.nfreq, p.nb, p.nv, res.base);
     GPP_TRY(hipGetLastError());
     if (timer) timer->mark(s, "factored_final");
     return hipSuccess;
