@@ -32,6 +32,8 @@ NB, NV, NCOULS, NGPOWN, NFREQ, SEED = 512, 2, 32768, 1638, 3, 1
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix, dense
 PEAK_HBM_GBS = 8000.0
+PEAK_FP16_TFLOPS = 2500.0 # dense FP16 MFMA
+PEAK_INT8_TOPS = 5000.0   # dense INT8 MFMA
 FLOP_UNOCC = 33           # per unoccupied update, counted on the reference's expression
 FLOP_OCC = 62             # per occupied update (branch-1 path)
 
@@ -121,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--form", default="factored-f64",
-                    choices=["direct", "direct-strict", "factored-f64", "factored-ozaki"])
+                    choices=["direct", "direct-strict", "factored-f64", "factored-ozaki", "factored-ozaki-i8"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -138,7 +140,8 @@ def main():
     ctx = pygpp.Context(0)
     ctx.generate_seeded(prob, SEED)
 
-    run_forms = [args.form] + [f for f in ("factored-f64", "direct", "direct-strict") if f != args.form]
+    run_forms = [args.form] + [f for f in ("factored-f64", "factored-ozaki-i8", "factored-ozaki", "direct",
+                                           "direct-strict") if f != args.form]
     forms = {}
     results = {}
     for form in run_forms:
@@ -153,8 +156,9 @@ def main():
     for form, out in results.items():
         keys = ("achtemp", "asxtemp", "achstemp") + (("acht_n1_loc",) if out.get("acht_n1_loc") is not None else ())
         forms[form]["max_rel_err_vs_direct_strict"] = max(rel_err(out[k], base[k]) for k in keys)
-    if "factored-f64" in forms:
-        forms["factored-f64"]["speedup_vs_direct"] = forms["direct"]["ms_per_step"] / forms["factored-f64"]["ms_per_step"]
+    for f in forms:
+        if f.startswith("factored"):
+            forms[f]["speedup_vs_direct"] = forms["direct"]["ms_per_step"] / forms[f]["ms_per_step"]
 
     def direct_roofline(form):
         kname = "direct_unoccupied_strict" if form == "direct-strict" else "direct_unoccupied"
@@ -170,15 +174,25 @@ def main():
 
     def factored_roofline(form):
         st = forms[form]["gpu_stage_ms"]
-        gms = st["factored_gram_f64"]
-        flop = 8 * NGPOWN * NCOULS * (NB - NV)
+        gkey = [k for k in st if k.startswith("factored_gram")][0]
+        gms = st[gkey]
+        K = NB - NV
+        if form == "factored-f64":
+            flop = 8 * NGPOWN * NCOULS * K
+            peak, pname, tr = PEAK_FP64_TFLOPS, "v_mfma_f64_16x16x4", load_traffic("k_gram_f64")
+        else:
+            # executed Ozaki work: 21 digit pairs x 4 real products, split included in the stage time
+            flop = 21 * 4 * 2 * NGPOWN * NCOULS * K
+            i8 = form.endswith("i8")
+            peak = PEAK_INT8_TOPS if i8 else PEAK_FP16_TFLOPS
+            pname = "v_mfma_i32_16x16x64_i8" if i8 else "v_mfma_f32_16x16x32_f16"
+            tr = None
         achieved = flop / (gms * 1e-3) / 1e12
-        tr = load_traffic("k_gram_f64")
         red_ms = st["factored_pole_reduce"]
         red_bytes = 3 * 16 * NGPOWN * NCOULS  # wtilde, I_eps, Mu
         tr2 = load_traffic("k_pole_reduce")
-        return {"bound": "mfma", "kernel": "factored_gram_f64 (v_mfma_f64_16x16x4)", "achieved": achieved,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+        return {"bound": "mfma", "kernel": f"{gkey} ({pname})", "achieved": achieved,
+                "peak": peak, "unit": "TFLOP/s" if not form.endswith("i8") else "TOP/s", "frac": achieved / peak,
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
                 "algorithmic_flop_per_launch": flop, "avg_launch_ms": gms,
                 "second_kernel": {"bound": "hbm", "kernel": "factored_pole_reduce", "avg_launch_ms": red_ms,

@@ -157,6 +157,8 @@ struct gpp_ctx {
     gpp::Workspace work;
     gpp::StageTimer timer;
     bool timed = false;
+    int ozaki_slices = 6;
+    int ozaki_layout = 1;
 };
 
 namespace {
@@ -197,6 +199,15 @@ gpp_status ensure_owned(gpp_ctx *ctx, const gpp_problem *p)
     dp.ieps = static_cast<const cplx *>(ctx->ieps.ptr);
     dp.wtilde = static_cast<const cplx *>(ctx->wtilde.ptr);
     return GPP_OK;
+}
+
+gpp::GramKind gram_kind(const gpp_ctx *ctx, gpp_form form)
+{
+    gpp::GramKind k;
+    if (form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) k.ozaki_slices = ctx->ozaki_slices;
+    k.int8 = form == GPP_FORM_FACTORED_OZAKI_I8;
+    k.layout = ctx->ozaki_layout;
+    return k;
 }
 
 gpp_status check_index_device(gpp_ctx *ctx, const int32_t *d_idx, int ngpown)
@@ -440,7 +451,8 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (!out || !out->achtemp || !out->asxtemp)
         return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
     if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
-    if ((form == GPP_FORM_FACTORED_F64 || form == GPP_FORM_FACTORED_OZAKI) && out->acht_n1_loc)
+    if ((form == GPP_FORM_FACTORED_F64 || form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) &&
+        out->acht_n1_loc)
         return fail(GPP_ERR_UNSUPPORTED, "acht_n1_loc is a per-band sum and comes from the direct forms only; "
                                          "pass NULL with the factored forms");
     gpp_status st = set_device(ctx);
@@ -450,8 +462,9 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     switch (form) {
     case GPP_FORM_DIRECT:
     case GPP_FORM_DIRECT_STRICT: need = gpp::direct_workspace_bytes(dp); break;
-    case GPP_FORM_FACTORED_F64: need = gpp::factored_workspace_bytes(dp, false); break;
-    case GPP_FORM_FACTORED_OZAKI: need = gpp::factored_workspace_bytes(dp, true); break;
+    case GPP_FORM_FACTORED_F64:
+    case GPP_FORM_FACTORED_OZAKI:
+    case GPP_FORM_FACTORED_OZAKI_I8: need = gpp::factored_workspace_bytes(dp, gram_kind(ctx, form)); break;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
     }
     if (need == 0) return fail(GPP_ERR_UNSUPPORTED, "form %d is not available for this problem", (int)form);
@@ -465,7 +478,7 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT)
         e = gpp::run_direct(dp, form == GPP_FORM_DIRECT_STRICT, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     else
-        e = gpp::run_factored(dp, form == GPP_FORM_FACTORED_OZAKI, ctx->work.ptr, res, ctx->stream, &ctx->timer);
+        e = gpp::run_factored(dp, gram_kind(ctx, form), ctx->work.ptr, res, ctx->stream, &ctx->timer);
     if (e != hipSuccess) return fail_hip(e, "sigma launch");
     std::vector<cplx> host(res.count());
     HIP_OR_FAIL(hipMemcpyAsync(host.data(), res.base, sizeof(cplx) * res.count(), hipMemcpyDeviceToHost, ctx->stream),
@@ -477,6 +490,32 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (out->acht_n1_loc) memcpy(out->acht_n1_loc, host.data() + 2 * dp.nfreq, sizeof(cplx) * dp.nb);
     if (out->achstemp) memcpy(out->achstemp, host.data() + 2 * dp.nfreq + dp.nb, sizeof(cplx));
     return GPP_OK;
+}
+
+gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    switch (option) {
+    case GPP_OPT_OZAKI_SLICES:
+        if (value < 3 || value > 6) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki slices must be 3..6, got %d", value);
+        ctx->ozaki_slices = value;
+        return GPP_OK;
+    case GPP_OPT_OZAKI_LAYOUT:
+        if (value < 0 || value > 1) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0 or 1, got %d", value);
+        ctx->ozaki_layout = value;
+        return GPP_OK;
+    default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
+    }
+}
+
+gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
+{
+    if (!ctx || !value) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    switch (option) {
+    case GPP_OPT_OZAKI_SLICES: *value = ctx->ozaki_slices; return GPP_OK;
+    case GPP_OPT_OZAKI_LAYOUT: *value = ctx->ozaki_layout; return GPP_OK;
+    default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
+    }
 }
 
 gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *stage_ms, int max_stages,

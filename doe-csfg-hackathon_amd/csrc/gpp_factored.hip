@@ -305,16 +305,16 @@ void launch_reduce_fc(const ReduceArgs &a, hipStream_t s)
 
 }  // namespace
 
-size_t factored_workspace_bytes(const DevProblem &p, bool ozaki)
+size_t factored_workspace_bytes(const DevProblem &p, GramKind kind)
 {
-    if (ozaki) return 0;  // FP16 Ozaki slices: not built
-    return fact_layout(p).total;
+    size_t n = fact_layout(p).total;
+    if (kind.ozaki_slices) n += ozaki_workspace_bytes(p, kind.ozaki_slices, kind.int8);
+    return n;
 }
 
-hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevResults &res, hipStream_t s,
+hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const DevResults &res, hipStream_t s,
                         StageTimer *timer)
 {
-    if (ozaki) return hipErrorNotSupported;
     const FactLayout L = fact_layout(p);
     char *w = static_cast<char *>(work);
     cplx *bg = reinterpret_cast<cplx *>(w + L.bg);
@@ -334,12 +334,15 @@ hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevRe
     }
     GPP_TRY(launch_achstemp(p, part_achs, s));
     if (timer) timer->mark(s, "factored_gather_achstemp");
-    {
+    if (kind.ozaki_slices) {
+        GPP_TRY(launch_gram_ozaki(p, kind.ozaki_slices, kind.int8, kind.layout, bg, w + L.total, mu, s));
+        if (timer) timer->mark(s, kind.int8 ? "factored_gram_ozaki_i8" : "factored_gram_ozaki");
+    } else {
         dim3 grid((p.ngpown + kTJ - 1) / kTJ, (p.ncouls + kTI - 1) / kTI);
         hipLaunchKernelGGL(k_gram_f64, grid, dim3(kGramThreads), 0, s, bg, p.aqsn, p.ngpown, p.ncouls, p.nv, p.nb, mu);
         GPP_TRY(hipGetLastError());
+        if (timer) timer->mark(s, "factored_gram_f64");
     }
-    if (timer) timer->mark(s, "factored_gram_f64");
     ReduceArgs ra;
     ra.mu = mu; ra.ieps = p.ieps; ra.wtilde = p.wtilde; ra.aqsn = p.aqsn; ra.bg = bg; ra.idx = p.idx;
     ra.vcoul = p.vcoul; ra.wx = p.wx; ra.nv = p.nv; ra.ncouls = p.ncouls; ra.ngpown = p.ngpown;

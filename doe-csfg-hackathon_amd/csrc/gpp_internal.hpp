@@ -89,9 +89,21 @@ hipError_t launch_accumulate_compat(const DevResults &res, int which, int nv, do
 hipError_t launch_fill_seeded(cplx *dst, size_t n, uint64_t seed, uint32_t array_id, double re_lo,
                               double re_hi, double im_lo, double im_hi, hipStream_t s);
 
-// Factored forms.
-size_t factored_workspace_bytes(const DevProblem &p, bool ozaki);
-hipError_t run_factored(const DevProblem &p, bool ozaki, void *work, const DevResults &res,
+// How the factored form computes its Gram product.
+struct GramKind {
+    int ozaki_slices = 0;  // 0: FP64 MFMA; 3..6: Ozaki split with that many slices
+    bool int8 = false;     // Ozaki digits in INT8 (else FP16)
+    int layout = 1;        // Ozaki wave layout (0: 2x2 waves of 32x32, 1: 2x4 waves of 32x16)
+};
+
+size_t factored_workspace_bytes(const DevProblem &p, GramKind kind);
+hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const DevResults &res,
                         hipStream_t s, StageTimer *timer);
+
+// Ozaki Gram product (gpp_ozaki.hip): Mu = conj(Bg)^T aqsn over bands
+// [nv, nb), written to mout [ngpown][ncouls].
+size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8);
+hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, const cplx *bg, void *work,
+                             cplx *mout, hipStream_t s);
 
 }  // namespace gpp

@@ -1,0 +1,353 @@
+// gpp_ozaki.hip -- the complex Gram product of the factored form as an
+// Ozaki-scheme split into FP16 slices, multiplied on FP16 MFMA with exact FP32
+// accumulation, recombined in FP64.
+//
+//   Mu[j][ig] = sum_{k in [nv, nb)} conj(Bg[k][j]) * aqsn[k][ig]
+//
+// Splitting (error-free).  Every column c of an operand (a G' column j of Bg,
+// a G-vector ig of aqsn) gets a power-of-two scale 2^e_c > max_k |re|, |im|.
+// Each real part x is written as
+//   x = 2^e_c * sum_{p<S} d_p 2^{-6-7p} + r,   d_p integer in [-64, 64],
+//   |r| <= 2^(e_c - 7S)
+// (d_0 = rint(x 2^(6-e_c)), each later digit rounds 2^7 times the previous
+// remainder).  Digits are exact in FP16.  The product of digits p and q has
+// weight 2^(e_j + e_ig - 12 - 7(p+q)); all pairs with the same level t = p+q
+// share one FP32 accumulator, and pairs with t >= S are dropped (their weight
+// is below the split's own truncation).
+//
+// Exactness.  A level-t accumulator receives (t+1) pairs x 2 real products
+// (conj(b)a: br ar + bi ai, br ai - bi ar) of magnitude <= 64*64 per band, so
+// over a chunk of 256 bands |sum| <= 6 * 2 * 4096 * 256 < 2^24 for
+// S <= 6: every partial sum is an integer below 2^24 and FP32 holds it
+// exactly, whatever order the MFMA adds in.  After each chunk the levels are
+// converted to FP64 and accumulated.
+//
+// The same kernel runs with INT8 digits (v_mfma_i32_16x16x64_i8): digits in
+// [-64, 64] are exact int8, products accumulate exactly in INT32 while
+// 6 * 2 * 4096 * K < 2^31 (K < 43690 bands), so no chunking is needed; MI355X
+// runs INT8 MFMA at twice the FP16 rate and the staged digits are half the bytes.
+//
+// MFMA lane layouts (measured on gfx950), 16 bytes of operand per lane:
+//   f32_16x16x32_f16: A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]
+//   i32_16x16x64_i8 : A[l%16][16(l/16)+e], B[16(l/16)+e][l%16]
+//   both            : result r of lane l = C[4(l/16)+r][l%16].
+#include "gpp_internal.hpp"
+
+namespace gpp {
+namespace {
+
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef signed char v16c __attribute__((ext_vector_type(16)));
+
+constexpr int kOzTile = 64;     // block tile: 64 G' columns x 64 G-vectors
+constexpr int kOzRowBytes = 80; // LDS row pitch: 64 bytes of digits + 16 pad
+
+// Wave layouts of the 64 x 64 block tile: NWJ x NWI waves, each owning
+// SJ x SI sub-tiles of 16 x 16 (NWJ*SJ*16 = NWI*SI*16 = 64).
+//   layout 0: 2 x 2 waves of 32 x 32 (one wave per SIMD, 512 registers each)
+//   layout 1: 2 x 4 waves of 32 x 16 (two waves per SIMD, fewer accumulators)
+template <int L> struct OzWaves;
+template <> struct OzWaves<0> { static constexpr int NWJ = 2, NWI = 2, SJ = 2, SI = 2; };
+template <> struct OzWaves<1> { static constexpr int NWJ = 2, NWI = 4, SJ = 2, SI = 1; };
+
+// Digit-type traits: FP16 digits on f32_16x16x32_f16 with exact FP32 sums over
+// chunks of 256 bands; INT8 digits on i32_16x16x64_i8 with exact INT32 sums.
+template <typename D> struct OzDigit;
+template <> struct OzDigit<_Float16> {
+    typedef v8h frag;
+    typedef v4f acc;
+    static constexpr int KS = 32;       // bands per MFMA k-step = per LDS stage (64 bytes)
+    static constexpr int CHUNK = 256;   // bands per exact accumulation chunk
+    static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ frag neg(frag x) { return -x; }
+};
+template <> struct OzDigit<int8_t> {
+    typedef v4i frag;
+    typedef v4i acc;
+    static constexpr int KS = 64;
+    static constexpr int CHUNK = 32768;
+    static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
+    {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ frag neg(frag x)
+    {
+        v16c b = __builtin_bit_cast(v16c, x);
+        return __builtin_bit_cast(v4i, -b);
+    }
+};
+
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+__host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Column exponent e with 2^e > max over rows [k_begin, k_end) of |re|, |im| of
+// column c of a row-major complex matrix (row stride ld).  Zero columns get 0.
+__global__ void k_oz_colexp(const cplx *__restrict__ src, int ld, int k_begin, int k_end, int ncols, int ncols_pad,
+                            int *__restrict__ exps)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols_pad) return;
+    double m = 0.0;
+    if (c < ncols)
+        for (int k = k_begin; k < k_end; ++k) {
+            const cplx v = src[(size_t)k * ld + c];
+            m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
+        }
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);  // m = f 2^e, f in [0.5, 1): 2^e > m
+    exps[c] = e;
+}
+
+// Digits of column c, bands [k_begin + 8*kg, +8), into
+// out[((p*2 + part) * ncols_pad + c) * kpad + k] (k contiguous).
+template <int S, typename D>
+__global__ void k_oz_split(const cplx *__restrict__ src, int ld, int k_begin, int K, int ncols, int ncols_pad,
+                           int kpad, const int *__restrict__ exps, D *__restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kg = blockIdx.y;
+    if (c >= ncols_pad) return;
+    const int e = exps[c];
+    D dre[S][8], dim[S][8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        const int k = kg * 8 + kk;
+        cplx v = czero();
+        if (c < ncols && k < K) v = src[(size_t)(k_begin + k) * ld + c];
+        double yr = ldexp(v.re, 6 - e), yi = ldexp(v.im, 6 - e);  // |y| < 64, exact
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            const double dr = rint(yr), di = rint(yi);
+            dre[p][kk] = (D)dr;
+            dim[p][kk] = (D)di;
+            yr = (yr - dr) * 128.0;  // exact: |yr - dr| <= 0.5
+            yi = (yi - di) * 128.0;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        __builtin_memcpy(out + ((size_t)(p * 2 + 0) * ncols_pad + c) * kpad + kg * 8, dre[p], 8 * sizeof(D));
+        __builtin_memcpy(out + ((size_t)(p * 2 + 1) * ncols_pad + c) * kpad + kg * 8, dim[p], 8 * sizeof(D));
+    }
+}
+
+// Mu tile from the digit planes.  Grid (ngpown_pad/64, ncouls_pad/64), 64*NWJ*NWI threads.
+template <int S, typename D, int LAYOUT>
+__global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad,
+                                                            const int *__restrict__ eb, const int *__restrict__ ea,
+                                                            int ngpown, int ncouls, int ngp_pad, int nc_pad, int kpad,
+                                                            cplx *__restrict__ mout)
+{
+    typedef OzDigit<D> T;
+    typedef OzWaves<LAYOUT> W;
+    constexpr int SJ = W::SJ, SI = W::SI, NTH = 64 * W::NWJ * W::NWI;
+    typedef typename T::frag frag;
+    typedef typename T::acc acc_t;
+    constexpr int KS = T::KS;
+    __shared__ __attribute__((aligned(16))) unsigned char sB[S * 2][kOzTile][kOzRowBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sA[S * 2][kOzTile][kOzRowBytes];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int wj = (wave / W::NWI) * (16 * SJ), wi = (wave % W::NWI) * (16 * SI);
+    const int j0 = blockIdx.x * kOzTile, i0 = blockIdx.y * kOzTile;
+    const int fo = 16 * (lane / 16);  // fragment byte offset within a 64-byte row
+
+    acc_t acc_re[S][SJ][SI], acc_im[S][SJ][SI];
+    double m_re[SJ][SI][4], m_im[SJ][SI][4];
+#pragma unroll
+    for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+        for (int si = 0; si < SI; ++si) {
+#pragma unroll
+            for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) m_re[sj][si][r] = m_im[sj][si][r] = 0.0;
+        }
+
+    // Fold the level accumulators into the FP64 sums (smallest level first).
+    auto flush = [&]() {
+#pragma unroll
+        for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+            for (int si = 0; si < SI; ++si)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double sr = 0.0, sm = 0.0;
+#pragma unroll
+                    for (int t = S - 1; t >= 0; --t) {
+                        sr = sr * 0x1p-7 + (double)acc_re[t][sj][si][r];
+                        sm = sm * 0x1p-7 + (double)acc_im[t][sj][si][r];
+                    }
+                    m_re[sj][si][r] += sr;
+                    m_im[sj][si][r] += sm;
+                }
+#pragma unroll
+        for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+            for (int si = 0; si < SI; ++si)
+#pragma unroll
+                for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
+    };
+
+    const size_t rowb = (size_t)kpad * sizeof(D);  // bytes per digit row in HBM
+    const unsigned char *gb = reinterpret_cast<const unsigned char *>(bd);
+    const unsigned char *ga = reinterpret_cast<const unsigned char *>(ad);
+    // 16-byte staging units per stage: 2S planes x 64 rows x 4 per side.
+    constexpr int UNITS = S * 2 * kOzTile * 4;
+    for (int k0 = 0; k0 < kpad; k0 += KS) {
+        const size_t kb = (size_t)k0 * sizeof(D);
+        for (int u = threadIdx.x; u < 2 * UNITS; u += NTH) {
+            const int side = u / UNITS;
+            const int v = u % UNITS;
+            const int q4 = v % 4, row = (v / 4) % kOzTile, plane = v / (4 * kOzTile);  // plane = p*2 + part
+            const unsigned char *src = side ? ga : gb;
+            const int npad = side ? nc_pad : ngp_pad;
+            const int col = (side ? i0 : j0) + row;
+            const v4i val = *reinterpret_cast<const v4i *>(src + ((size_t)plane * npad + col) * rowb + kb + q4 * 16);
+            unsigned char *dst = side ? &sA[plane][row][q4 * 16] : &sB[plane][row][q4 * 16];
+            *reinterpret_cast<v4i *>(dst) = val;
+        }
+        __syncthreads();
+        // G-vector side fragments of every digit, kept in registers for the stage.
+        frag fa[S][SJ][SI];
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int part = 0; part < 2; ++part)
+#pragma unroll
+                for (int si = 0; si < SI; ++si)
+                    fa[q][part][si] = *reinterpret_cast<const frag *>(&sA[q * 2 + part][wi + 16 * si + lane % 16][fo]);
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            frag br[SJ], bi[SJ], bn[SJ];
+#pragma unroll
+            for (int sj = 0; sj < SJ; ++sj) {
+                br[sj] = *reinterpret_cast<const frag *>(&sB[p * 2 + 0][wj + 16 * sj + lane % 16][fo]);
+                bi[sj] = *reinterpret_cast<const frag *>(&sB[p * 2 + 1][wj + 16 * sj + lane % 16][fo]);
+                bn[sj] = T::neg(bi[sj]);
+            }
+#pragma unroll
+            for (int q = 0; q + p < S; ++q) {
+                const int t = p + q;
+                // conj(b) * a: re = br ar + bi ai, im = br ai - bi ar.  Each group of
+                // four MFMAs writes four different accumulators.
+#pragma unroll
+                for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(br[sj], fa[q][0][si], acc_re[t][sj][si]);
+#pragma unroll
+                for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(br[sj], fa[q][1][si], acc_im[t][sj][si]);
+#pragma unroll
+                for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(bi[sj], fa[q][1][si], acc_re[t][sj][si]);
+#pragma unroll
+                for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(bn[sj], fa[q][0][si], acc_im[t][sj][si]);
+            }
+        }
+        __syncthreads();
+        if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush();
+    }
+
+#pragma unroll
+    for (int sj = 0; sj < SJ; ++sj)
+#pragma unroll
+        for (int si = 0; si < SI; ++si)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = j0 + wj + 16 * sj + 4 * (lane / 16) + r;
+                const int ig = i0 + wi + 16 * si + lane % 16;
+                if (j < ngpown && ig < ncouls) {
+                    const int sc = eb[j] + ea[ig] - 12;
+                    mout[(size_t)j * ncouls + ig] = cplx{ldexp(m_re[sj][si][r], sc), ldexp(m_im[sj][si][r], sc)};
+                }
+            }
+}
+
+struct OzLayout {
+    size_t eb, ea, bd, ad, total;
+    int ngp_pad, nc_pad, kpad, K;
+};
+
+OzLayout oz_layout(const DevProblem &p, int slices, bool int8)
+{
+    const int ks = int8 ? OzDigit<int8_t>::KS : OzDigit<_Float16>::KS;
+    const size_t dsz = int8 ? 1 : 2;
+    OzLayout L;
+    L.K = p.nb - p.nv;
+    L.kpad = round_up(L.K > 0 ? L.K : 1, ks);
+    L.ngp_pad = round_up(p.ngpown, kOzTile);
+    L.nc_pad = round_up(p.ncouls, kOzTile);
+    size_t off = 0;
+    L.eb = off; off += align256(sizeof(int) * L.ngp_pad);
+    L.ea = off; off += align256(sizeof(int) * L.nc_pad);
+    L.bd = off; off += align256(dsz * (size_t)slices * 2 * L.ngp_pad * L.kpad);
+    L.ad = off; off += align256(dsz * (size_t)slices * 2 * L.nc_pad * L.kpad);
+    L.total = off;
+    return L;
+}
+
+template <int S, typename D>
+hipError_t launch_ozaki_s(const DevProblem &p, int layout, const cplx *bg, void *work, cplx *mout, hipStream_t s)
+{
+    const OzLayout L = oz_layout(p, S, sizeof(D) == 1);
+    char *w = static_cast<char *>(work);
+    int *eb = reinterpret_cast<int *>(w + L.eb);
+    int *ea = reinterpret_cast<int *>(w + L.ea);
+    D *bd = reinterpret_cast<D *>(w + L.bd);
+    D *ad = reinterpret_cast<D *>(w + L.ad);
+    hipLaunchKernelGGL(k_oz_colexp, dim3((L.ngp_pad + 255) / 256), dim3(256), 0, s, bg, p.ngpown, p.nv, p.nb,
+                       p.ngpown, L.ngp_pad, eb);
+    hipLaunchKernelGGL(k_oz_colexp, dim3((L.nc_pad + 255) / 256), dim3(256), 0, s, p.aqsn, p.ncouls, p.nv, p.nb,
+                       p.ncouls, L.nc_pad, ea);
+    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.ngp_pad + 255) / 256, L.kpad / 8), dim3(256), 0, s, bg,
+                       p.ngpown, p.nv, L.K, p.ngpown, L.ngp_pad, L.kpad, eb, bd);
+    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.nc_pad + 255) / 256, L.kpad / 8), dim3(256), 0, s, p.aqsn,
+                       p.ncouls, p.nv, L.K, p.ncouls, L.nc_pad, L.kpad, ea, ad);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid(L.ngp_pad / kOzTile, L.nc_pad / kOzTile);
+    if (layout == 1)
+        hipLaunchKernelGGL((k_gram_ozaki<S, D, 1>), grid, dim3(64 * OzWaves<1>::NWJ * OzWaves<1>::NWI), 0, s, bd, ad,
+                           eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, mout);
+    else
+        hipLaunchKernelGGL((k_gram_ozaki<S, D, 0>), grid, dim3(64 * OzWaves<0>::NWJ * OzWaves<0>::NWI), 0, s, bd, ad,
+                           eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, mout);
+    return hipGetLastError();
+}
+
+template <typename D>
+hipError_t launch_ozaki_d(const DevProblem &p, int slices, int layout, const cplx *bg, void *work, cplx *mout,
+                          hipStream_t s)
+{
+    switch (slices) {
+    case 3: return launch_ozaki_s<3, D>(p, layout, bg, work, mout, s);
+    case 4: return launch_ozaki_s<4, D>(p, layout, bg, work, mout, s);
+    case 5: return launch_ozaki_s<5, D>(p, layout, bg, work, mout, s);
+    case 6: return launch_ozaki_s<6, D>(p, layout, bg, work, mout, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8) { return oz_layout(p, slices, int8).total; }
+
+hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, const cplx *bg, void *work,
+                             cplx *mout, hipStream_t s)
+{
+    if (int8 && (p.nb - p.nv) >= 43690) return hipErrorInvalidValue;  // INT32 exactness bound
+    return int8 ? launch_ozaki_d<int8_t>(p, slices, layout, bg, work, mout, s)
+                : launch_ozaki_d<_Float16>(p, slices, layout, bg, work, mout, s);
+}
+
+}  // namespace gpp

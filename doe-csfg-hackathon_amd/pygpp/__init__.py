@@ -32,12 +32,17 @@ GPP_ERR_LAUNCH = 5
 GPP_ERR_UNSUPPORTED = 6
 GPP_ERR_NO_INPUTS = 7
 
+# gpp_option
+GPP_OPT_OZAKI_SLICES = 1
+GPP_OPT_OZAKI_LAYOUT = 2
+
 # gpp_form
 FORMS = {
     "direct": 0,
     "direct-strict": 1,
     "factored-f64": 2,
     "factored-ozaki": 3,
+    "factored-ozaki-i8": 4,
 }
 
 # Every symbol include/gpp.h declares.
@@ -52,6 +57,8 @@ EXPORTED_SYMBOLS = (
     "gpp_fill_seeded",
     "gpp_generate_seeded",
     "gpp_sigma",
+    "gpp_ctx_set_option",
+    "gpp_ctx_get_option",
     "gpp_ctx_last_timing",
     "gpp_ctx_stage_name",
     "d_till_nvband",
@@ -112,6 +119,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_generate_seeded.argtypes = [vp, P(GppProblem), ctypes.c_uint64, dbl, dbl]
     lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
     lib.gpp_ctx_last_timing.argtypes = [vp, P(dbl), P(dbl), i32, P(i32)]
+    lib.gpp_ctx_set_option.argtypes = [vp, i32, i32]
+    lib.gpp_ctx_get_option.argtypes = [vp, i32, P(i32)]
     lib.gpp_ctx_stage_name.restype = ctypes.c_char_p
     lib.gpp_ctx_stage_name.argtypes = [vp, i32]
     lib.d_till_nvband.argtypes = [i32, i32, i32] + [vp] * 11 + [dbl]
@@ -119,7 +128,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.d_noFlagOCCSolver.argtypes = [i32, i32, i32, i32] + [vp] * 10
     for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device",
                  "gpp_fill_deterministic", "gpp_fill_seeded", "gpp_generate_seeded", "gpp_sigma",
-                 "gpp_ctx_last_timing", "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
+                 "gpp_ctx_last_timing", "gpp_ctx_set_option", "gpp_ctx_get_option", "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
         getattr(lib, name).restype = i32
     _lib = lib
     return lib
@@ -257,6 +266,14 @@ class Context:
         _check(load().gpp_sigma(self._h, FORMS[form], ctypes.byref(o)))
         out["achstemp"] = out["achstemp"][0]
         return out
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(load().gpp_ctx_set_option(self._h, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        _check(load().gpp_ctx_get_option(self._h, option, ctypes.byref(v)))
+        return v.value
 
     def last_timing(self) -> Dict[str, float]:
         """GPU milliseconds of the last sigma: 'total' and one entry per stage."""

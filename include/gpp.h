@@ -85,10 +85,27 @@ typedef enum {
      * followed by a fused FP64 pole-factor reduce. */
     GPP_FORM_FACTORED_F64 = 2,
     /* As GPP_FORM_FACTORED_F64 with the Gram product split into FP16 slices
-     * (Ozaki scheme) multiplied on FP16 MFMA with FP32 accumulation and
-     * recombined in FP64. */
-    GPP_FORM_FACTORED_OZAKI = 3
+     * (Ozaki scheme: 7-bit digits under a power-of-two scale per column),
+     * multiplied on FP16 MFMA with exact FP32 accumulation and recombined in
+     * FP64; slices per operand from GPP_OPT_OZAKI_SLICES. */
+    GPP_FORM_FACTORED_OZAKI = 3,
+    /* As GPP_FORM_FACTORED_OZAKI with the same digits held in INT8 and
+     * multiplied on INT8 MFMA with exact INT32 accumulation (needs
+     * number_bands - nvband < 43690). */
+    GPP_FORM_FACTORED_OZAKI_I8 = 4
 } gpp_form;
+
+/* Tunable options of a context (gpp_ctx_set_option / gpp_ctx_get_option). */
+typedef enum {
+    /* Slices per operand of GPP_FORM_FACTORED_OZAKI, 3..6 (default 6).  Each
+     * slice carries 7 bits; the split truncates at about 2^(-7 S) of the
+     * column scale. */
+    GPP_OPT_OZAKI_SLICES = 1,
+    /* Wave layout of the Ozaki Gram kernel's 64x64 block tile: 0 = 2x2 waves
+     * of 32x32 (one wave per SIMD), 1 = 2x4 waves of 32x16 (two per SIMD,
+     * default). */
+    GPP_OPT_OZAKI_LAYOUT = 2
+} gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).
  * achtemp and asxtemp have nfreq entries; acht_n1_loc has number_bands
@@ -186,6 +203,10 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
  * call is available from gpp_ctx_last_timing.
  */
 gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out);
+
+/* Sets / reads a context option (gpp_option). */
+gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value);
+gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value);
 
 /*
  * GPU time of the last gpp_sigma in milliseconds, measured with HIP events

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 TOL_FACTORED = 1e-10
 KEYS = ("achtemp", "asxtemp", "achstemp")
-FACTORED = ["factored-f64"]
+FACTORED = ["factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 SHAPES = [
     (16, 2, 1000, 37, 3),
@@ -71,13 +71,64 @@ def test_factored_full_c1_band_subset(ctx, oracle):
 
 
 def test_factored_full_c1_vs_direct_strict(ctx):
-    """Full benchmark size: factored FP64 against the strict direct kernel."""
+    """Full benchmark size: every factored form against the strict direct kernel."""
     import pygpp
     p = pygpp.Problem(512, 2, 32768, 1638)
     ctx.generate_seeded(p, seed=1)
     ref = ctx.sigma("direct-strict")
-    out = ctx.sigma("factored-f64")
-    assert_close(out, ref, TOL_FACTORED, KEYS)
-    again = ctx.sigma("factored-f64")
-    for k in KEYS:
-        np.testing.assert_array_equal(out[k], again[k])
+    for form in FACTORED:
+        out = ctx.sigma(form)
+        assert_close(out, ref, TOL_FACTORED, KEYS)
+        again = ctx.sigma(form)
+        for k in KEYS:
+            np.testing.assert_array_equal(out[k], again[k])
+
+
+@pytest.mark.parametrize("form", ["factored-ozaki", "factored-ozaki-i8"])
+def test_ozaki_error_falls_with_slices(ctx, oracle, form):
+    """Each slice carries 7 more bits: the error must fall by roughly 2^7 per
+    slice, and 6 slices must meet the 1e-10 bound."""
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(40, 1200, 50, seed=21)
+    ctx.upload(problem_of(arrs, 2), arrs)
+    ref = oracle.sigma(arrs, nvband=2, threads=8)
+    errs = []
+    try:
+        for s in (3, 4, 5, 6):
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, s)
+            errs.append(comp_rel_err(ctx.sigma(form)["achtemp"], ref["achtemp"]))
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, 6)
+    assert errs[3] <= TOL_FACTORED
+    for a, b in zip(errs, errs[1:]):
+        assert b < a / 16, errs
+
+
+def test_ozaki_layouts_and_digit_types_agree(ctx):
+    """Both wave layouts compute the same exact integer sums; FP16 and INT8
+    digits differ only in where FP64 rounding of the exact sums happens."""
+    import pygpp
+    arrs = gpp_inputs.near_pole_inputs(30, 700, 70, seed=22)
+    ctx.upload(problem_of(arrs, 3), arrs)
+    out = {}
+    try:
+        for lay in (0, 1):
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, lay)
+            for form in ("factored-ozaki", "factored-ozaki-i8"):
+                out[(form, lay)] = ctx.sigma(form)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 1)
+    for form in ("factored-ozaki", "factored-ozaki-i8"):
+        for k in KEYS:
+            np.testing.assert_array_equal(out[(form, 0)][k], out[(form, 1)][k])
+    assert_close(out[("factored-ozaki", 1)], out[("factored-ozaki-i8", 1)], 1e-13, KEYS)
+
+
+def test_ozaki_option_validation(ctx):
+    import pygpp
+    for bad in (2, 7):
+        with pytest.raises(pygpp.GppError):
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, bad)
+    with pytest.raises(pygpp.GppError):
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 5)
+    assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
