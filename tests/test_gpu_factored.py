@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import gpp_inputs
-from test_gpu_parity import assert_close, comp_rel_err, problem_of, run_gpu
+from test_gpu_parity import assert_close, assert_close_conditioned, comp_rel_err, problem_of, run_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -132,3 +132,51 @@ def test_ozaki_option_validation(ctx):
     with pytest.raises(pygpp.GppError):
         ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 5)
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
+
+
+ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
+
+
+def test_c0_cpu_config(ctx, oracle):
+    """The CPU-runnable configuration: 512 2 4096 10 (ngpown 409), the
+    reference's deterministic init, every form against the oracle; also the
+    ngpown=410 and zero-based index variants."""
+    for ngpown, zero in ((409, False), (410, True)):
+        arrs = gpp_inputs.det_inputs(512, 2, 4096, ngpown=ngpown, igp_zero=zero)
+        ref = oracle.sigma(arrs, nvband=2, threads=16)
+        ctx.upload(problem_of(arrs, 2), arrs)
+        for form in ALL_FORMS:
+            tol = 1e-12 if form == "direct" else TOL_FACTORED
+            keys = KEYS + (("acht_n1_loc",) if form == "direct" else ())
+            assert_close(ctx.sigma(form), ref, tol, keys)
+
+
+def test_c3_shape_subset_vs_oracle(ctx, oracle):
+    """The large single-instance shape (ncouls 65536, nfreq 16 on the
+    4 + 0.25 (iw - 7.5) grid, 8 occupied bands) on a subset of bands and
+    columns small enough for the oracle.  Some achtemp components cancel
+    strongly here (the reference's own left-to-right sum is 2.3e-12 off the
+    exact sum of its terms on one of them), so parity is judged against the
+    compensated oracle and the sequential oracle's measured error."""
+    import pygpp
+    p = pygpp.Problem(12, 8, 65536, 48, nfreq=16)
+    arrs = pygpp.fill_seeded(p, seed=33, wx_center=4.0, dw=0.25)
+    seq = oracle.sigma(arrs, nvband=8, threads=16)
+    dd = oracle.sigma_dd(arrs, nvband=8, threads=16)
+    ctx.upload(p, arrs)
+    for form in ALL_FORMS:
+        tol = 1e-12 if form == "direct" else TOL_FACTORED
+        out = ctx.sigma(form)
+        assert_close_conditioned(out, seq, dd, tol)
+        assert_close(out, seq, tol, ("achstemp",) + (("acht_n1_loc",) if form == "direct" else ()))
+
+
+def test_c3_full_size_forms_agree(ctx):
+    """Full large instance (1536, 8, 65536, 3276, nfreq 16, ~10 GB resident):
+    the factored forms against the direct FP64 kernel."""
+    import pygpp
+    p = pygpp.Problem(1536, 8, 65536, 3276, nfreq=16)
+    ctx.generate_seeded(p, seed=3, wx_center=4.0, dw=0.25)
+    ref = ctx.sigma("direct")
+    for form in ("factored-f64", "factored-ozaki-i8"):
+        assert_close(ctx.sigma(form), ref, TOL_FACTORED, KEYS)

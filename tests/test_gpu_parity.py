@@ -36,6 +36,20 @@ def assert_close(out, ref, tol=TOL, keys=("achtemp", "asxtemp", "acht_n1_loc", "
         assert e <= tol, f"{k}: rel err {e:.3e} > {tol:.1e}\n got {out[k]}\n ref {ref[k]}"
 
 
+def assert_close_conditioned(out, seq, dd, tol=TOL, keys=("achtemp", "asxtemp")):
+    """For ill-conditioned sums: within tol of the exactly summed reference
+    terms (dd, the compensated oracle), and within tol of the sequential
+    reference plus that reference's own summation error |seq - dd|."""
+    for k in keys:
+        g = np.atleast_1d(np.asarray(out[k], np.complex128))
+        s_ = np.atleast_1d(np.asarray(seq[k], np.complex128))
+        d = np.atleast_1d(np.asarray(dd[k], np.complex128))
+        assert comp_rel_err(g, d) <= tol, f"{k}: vs compensated oracle {comp_rel_err(g, d):.3e}"
+        for gc, sc, dc in ((g.real, s_.real, d.real), (g.imag, s_.imag, d.imag)):
+            bound = tol * np.maximum(np.abs(sc), 1e-6 * np.abs(d)) + np.abs(sc - dc)
+            assert np.all(np.abs(gc - sc) <= bound), f"{k}: {np.abs(gc - sc)} > {bound}"
+
+
 def problem_of(arrs, nv):
     import pygpp
     nb, ncouls = arrs["aqsmtemp"].shape
