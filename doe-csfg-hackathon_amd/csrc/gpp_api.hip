@@ -4,8 +4,11 @@
 // dispatch to the kernel launchers of gpp_direct.hip / gpp_factored.hip.
 // Nothing here prints, exits or falls back to a CPU path: every failure is a
 // gpp_status plus a thread-local message.
+#include <algorithm>
+#include <map>
 #include <mutex>
 #include <new>
+#include <tuple>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -159,6 +162,12 @@ struct gpp_ctx {
     bool timed = false;
     int ozaki_slices = 6;
     int ozaki_layout = 1;
+    int gram_tile = 0;
+    int red_chunk = 8192;
+    int red_fc = 8;
+    int num_cus = 256;
+    // autotune results: (form, nb, nv, ncouls, ngpown, nfreq) -> (tile, chunk, fc)
+    std::map<std::tuple<int, int, int, int, int, int>, std::tuple<int, int, int>> tuned;
 };
 
 namespace {
@@ -207,6 +216,9 @@ gpp::GramKind gram_kind(const gpp_ctx *ctx, gpp_form form)
     if (form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) k.ozaki_slices = ctx->ozaki_slices;
     k.int8 = form == GPP_FORM_FACTORED_OZAKI_I8;
     k.layout = ctx->ozaki_layout;
+    k.tile = ctx->gram_tile;
+    k.red_chunk = ctx->red_chunk;
+    k.red_fc = ctx->red_fc;
     return k;
 }
 
@@ -248,6 +260,7 @@ gpp_status gpp_ctx_create(gpp_ctx **out, int device)
     gpp_ctx *ctx = new (std::nothrow) gpp_ctx();
     if (!ctx) return fail(GPP_ERR_OUT_OF_MEMORY, "host allocation failed");
     ctx->device = device;
+    ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = ctx->timer.init();
@@ -504,6 +517,20 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         if (value < 0 || value > 1) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0 or 1, got %d", value);
         ctx->ozaki_layout = value;
         return GPP_OK;
+    case GPP_OPT_GRAM_TILE:
+        if (value < 0 || value >= gpp::kGramTiles)
+            return fail(GPP_ERR_INVALID_ARGUMENT, "gram tile must be 0..%d, got %d", gpp::kGramTiles - 1, value);
+        ctx->gram_tile = value;
+        return GPP_OK;
+    case GPP_OPT_REDUCE_CHUNK:
+        if (value < 256 || value > (1 << 20) || (value & (value - 1)))
+            return fail(GPP_ERR_INVALID_ARGUMENT, "reduce chunk must be a power of two in [256, 2^20], got %d", value);
+        ctx->red_chunk = value;
+        return GPP_OK;
+    case GPP_OPT_REDUCE_FREQS:
+        if (value != 4 && value != 8) return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs must be 4 or 8, got %d", value);
+        ctx->red_fc = value;
+        return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
 }
@@ -514,8 +541,78 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
     switch (option) {
     case GPP_OPT_OZAKI_SLICES: *value = ctx->ozaki_slices; return GPP_OK;
     case GPP_OPT_OZAKI_LAYOUT: *value = ctx->ozaki_layout; return GPP_OK;
+    case GPP_OPT_GRAM_TILE: *value = ctx->gram_tile; return GPP_OK;
+    case GPP_OPT_REDUCE_CHUNK: *value = ctx->red_chunk; return GPP_OK;
+    case GPP_OPT_REDUCE_FREQS: *value = ctx->red_fc; return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
+}
+
+gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT) return GPP_OK;
+    if (form != GPP_FORM_FACTORED_F64 && form != GPP_FORM_FACTORED_OZAKI && form != GPP_FORM_FACTORED_OZAKI_I8)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
+    if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    const gpp::DevProblem &dp = ctx->dp;
+    const auto key = std::make_tuple((int)form, dp.nb, dp.nv, dp.ncouls, dp.ngpown, dp.nfreq);
+    auto hit = ctx->tuned.find(key);
+    if (hit != ctx->tuned.end()) {
+        std::tie(ctx->gram_tile, ctx->red_chunk, ctx->red_fc) = hit->second;
+        return GPP_OK;
+    }
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    gpp::DevResults res;
+    res.nfreq = dp.nfreq;
+    res.nb = dp.nb;
+    HIP_OR_FAIL(ctx->results.ensure(sizeof(cplx) * res.count()), "results allocation");
+    res.base = static_cast<cplx *>(ctx->results.ptr);
+    // time one configuration: returns the Gram and reduce stage times (ms)
+    auto trial = [&](int tile, int chunk, int fc, double *gram_ms, double *red_ms) -> gpp_status {
+        gpp::GramKind k = gram_kind(ctx, form);
+        k.tile = tile;
+        k.red_chunk = chunk;
+        k.red_fc = fc;
+        HIP_OR_FAIL(ctx->work.reserve(gpp::factored_workspace_bytes(dp, k)), "workspace allocation");
+        double best_g = 1e300, best_r = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_OR_FAIL(gpp::run_factored(dp, k, ctx->work.ptr, res, ctx->stream, &ctx->timer), "autotune launch");
+            HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "autotune synchronize");
+            if (rep == 0) continue;  // warm-up
+            float ms = 0.f;
+            const gpp::StageTimer &t = ctx->timer;
+            for (int i = 0; i < t.n; ++i) {
+                HIP_OR_FAIL(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]), "hipEventElapsedTime");
+                if (strncmp(t.name[i], "factored_gram", 13) == 0) best_g = std::min(best_g, (double)ms);
+                if (strcmp(t.name[i], "factored_pole_reduce") == 0) best_r = std::min(best_r, (double)ms);
+            }
+        }
+        *gram_ms = best_g;
+        *red_ms = best_r;
+        return GPP_OK;
+    };
+    int best_tile = ctx->gram_tile, best_chunk = ctx->red_chunk, best_fc = ctx->red_fc;
+    double g, r, best = 1e300;
+    if (form == GPP_FORM_FACTORED_F64)
+        for (int tile = 0; tile < gpp::kGramTiles; ++tile) {
+            if ((st = trial(tile, best_chunk, best_fc, &g, &r))) return st;
+            if (g < best) best = g, best_tile = tile;
+        }
+    best = 1e300;
+    const int chunks[] = {2048, 4096, 8192, 16384};
+    for (int chunk : chunks)
+        for (int fc : {4, 8}) {
+            if (dp.nfreq <= 4 && fc == 8) continue;  // identical launches for nfreq <= 4
+            if ((st = trial(best_tile, chunk, fc, &g, &r))) return st;
+            if (r < best) best = r, best_chunk = chunk, best_fc = fc;
+        }
+    ctx->gram_tile = best_tile;
+    ctx->red_chunk = best_chunk;
+    ctx->red_fc = best_fc;
+    ctx->tuned[key] = std::make_tuple(best_tile, best_chunk, best_fc);
+    return GPP_OK;
 }
 
 gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *stage_ms, int max_stages,
@@ -544,6 +641,333 @@ const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i)
 {
     if (!ctx || i < 0 || i >= ctx->timer.n || !ctx->timer.name[i]) return "";
     return ctx->timer.name[i];
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches.
+struct gpp_batch {
+    gpp_ctx *ctx = nullptr;
+    int n = 0;
+    std::vector<gpp::DevProblem> dps;
+    std::vector<DevBuf> owned;  // library-owned inputs (7 per instance) for seeded batches
+    gpp::GramKind kind;
+    bool planned = false;
+    gpp::BatchPlan plan;
+    DevBuf work;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evs[3] = {nullptr, nullptr, nullptr};
+    double last_ms = 0.0;
+    double stage_ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int batch_blocks(const gpp_batch *b) { return 2 * b->ctx->num_cus; }
+
+void batch_drop_graph(gpp_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+gpp_status batch_prepare(gpp_batch *b, const gpp::GramKind &k)
+{
+    batch_drop_graph(b);
+    size_t need = 0;
+    b->plan = gpp::plan_batch(b->dps, k, &need);
+    HIP_OR_FAIL(b->work.ensure(need), "batch workspace allocation");
+    HIP_OR_FAIL(gpp::upload_batch(b->plan, b->dps, b->work.ptr, b->ctx->stream), "batch tables upload");
+    b->kind = k;
+    b->planned = true;
+    return GPP_OK;
+}
+
+gpp_status batch_capture(gpp_batch *b)
+{
+    hipStream_t s = b->ctx->stream;
+    HIP_OR_FAIL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e = gpp::launch_batch(b->plan, b->work.ptr, batch_blocks(b), s, 15);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail_hip(e, "batch capture launch");
+    }
+    if (e2 != hipSuccess) return fail_hip(e2, "hipStreamEndCapture");
+    b->graph = g;
+    HIP_OR_FAIL(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return GPP_OK;
+}
+
+gpp_status batch_new(gpp_ctx *ctx, int n, const gpp_problem *probs, gpp_batch **out, gpp_batch **made)
+{
+    if (!ctx || !out) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (n < 1 || !probs) return fail(GPP_ERR_INVALID_ARGUMENT, "a batch needs n >= 1 problems");
+    for (int i = 0; i < n; ++i) {
+        gpp_status st = check_problem(&probs[i]);
+        if (st) return st;
+    }
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    gpp_batch *b = new (std::nothrow) gpp_batch();
+    if (!b) return fail(GPP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    b->ctx = ctx;
+    b->n = n;
+    b->dps.resize(n);
+    for (int i = 0; i < n; ++i) fill_dp_sizes(b->dps[i], &probs[i]);
+    b->kind.tile = ctx->gram_tile;
+    b->kind.red_chunk = ctx->red_chunk;
+    b->kind.red_fc = ctx->red_fc;
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipEventCreate(&b->evs[0]) != hipSuccess || hipEventCreate(&b->evs[1]) != hipSuccess ||
+        hipEventCreate(&b->evs[2]) != hipSuccess) {
+        gpp_batch_destroy(b);
+        return fail(GPP_ERR_LAUNCH, "hipEventCreate failed");
+    }
+    *made = b;
+    return GPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gpp_status gpp_batch_create(gpp_ctx *ctx, int n, const gpp_problem *probs, const gpp_device_inputs *inputs,
+                            gpp_batch **out)
+{
+    if (!inputs) return fail(GPP_ERR_INVALID_ARGUMENT, "inputs is NULL");
+    gpp_batch *b = nullptr;
+    gpp_status st = batch_new(ctx, n, probs, out, &b);
+    if (st) return st;
+    for (int i = 0; i < n; ++i) {
+        const gpp_device_inputs &in = inputs[i];
+        if (!in.inv_igp_index || !in.wx_array || !in.vcoul || !in.aqsmtemp || !in.aqsntemp || !in.I_eps_array ||
+            !in.wtilde_array || !aligned16(in.aqsmtemp) || !aligned16(in.aqsntemp) || !aligned16(in.I_eps_array) ||
+            !aligned16(in.wtilde_array)) {
+            gpp_batch_destroy(b);
+            return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: NULL or misaligned device array", i);
+        }
+        if ((st = check_index_device(ctx, in.inv_igp_index, probs[i].ngpown))) {
+            gpp_batch_destroy(b);
+            return st;
+        }
+        gpp::DevProblem &dp = b->dps[i];
+        dp.idx = in.inv_igp_index;
+        dp.wx = in.wx_array;
+        dp.vcoul = in.vcoul;
+        dp.aqsm = reinterpret_cast<const cplx *>(in.aqsmtemp);
+        dp.aqsn = reinterpret_cast<const cplx *>(in.aqsntemp);
+        dp.ieps = reinterpret_cast<const cplx *>(in.I_eps_array);
+        dp.wtilde = reinterpret_cast<const cplx *>(in.wtilde_array);
+    }
+    *out = b;
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs, const uint64_t *seeds,
+                                   const double *wx_center, const double *dw, gpp_batch **out)
+{
+    if (!seeds) return fail(GPP_ERR_INVALID_ARGUMENT, "seeds is NULL");
+    gpp_batch *b = nullptr;
+    gpp_status st = batch_new(ctx, n, probs, out, &b);
+    if (st) return st;
+    b->owned.resize((size_t)7 * n);
+    hipStream_t s = ctx->stream;
+    for (int i = 0; i < n; ++i) {
+        const gpp_problem &p = probs[i];
+        if (p.ngpown > p.ncouls) {
+            gpp_batch_destroy(b);
+            return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: seeded inputs need ngpown <= ncouls", i);
+        }
+        const size_t nb = p.number_bands, nc = p.ncouls, ng = p.ngpown;
+        DevBuf *o = &b->owned[(size_t)7 * i];
+        const size_t bytes[7] = {sizeof(int32_t) * ng, sizeof(double) * p.nfreq, sizeof(double) * nc,
+                                 sizeof(cplx) * nb * nc, sizeof(cplx) * nb * nc, sizeof(cplx) * ng * nc,
+                                 sizeof(cplx) * ng * nc};
+        for (int q = 0; q < 7; ++q) {
+            hipError_t e = o[q].ensure(bytes[q]);
+            if (e != hipSuccess) {
+                gpp_batch_destroy(b);
+                return fail_hip(e, "batch input allocation");
+            }
+        }
+        std::vector<int32_t> idx(ng);
+        std::vector<double> wx(p.nfreq), vc(nc);
+        st = gpp_fill_seeded(&p, seeds[i], wx_center ? wx_center[i] : 4.0, dw ? dw[i] : 1.0, idx.data(), wx.data(),
+                             vc.data(), nullptr, nullptr, nullptr, nullptr);
+        hipError_t e = hipSuccess;
+        if (!st) e = hipMemcpyAsync(o[0].ptr, idx.data(), bytes[0], hipMemcpyHostToDevice, s);
+        if (!st && !e) e = hipMemcpyAsync(o[1].ptr, wx.data(), bytes[1], hipMemcpyHostToDevice, s);
+        if (!st && !e) e = hipMemcpyAsync(o[2].ptr, vc.data(), bytes[2], hipMemcpyHostToDevice, s);
+        if (!st && !e) e = gpp::launch_fill_seeded(static_cast<cplx *>(o[3].ptr), nb * nc, seeds[i], gpp::kArrAqsm, -1.0, 1.0, -1.0, 1.0, s);
+        if (!st && !e) e = gpp::launch_fill_seeded(static_cast<cplx *>(o[4].ptr), nb * nc, seeds[i], gpp::kArrAqsn, -1.0, 1.0, -1.0, 1.0, s);
+        if (!st && !e) e = gpp::launch_fill_seeded(static_cast<cplx *>(o[5].ptr), ng * nc, seeds[i], gpp::kArrIeps, -0.5, 0.5, -0.5, 0.5, s);
+        if (!st && !e) e = gpp::launch_fill_seeded(static_cast<cplx *>(o[6].ptr), ng * nc, seeds[i], gpp::kArrWtilde, 0.5, 5.0, -0.05, 0.05, s);
+        if (!st && !e) e = hipStreamSynchronize(s);  // the host vectors above go out of scope
+        if (st || e) {
+            gpp_batch_destroy(b);
+            return st ? st : fail_hip(e, "batch input generation");
+        }
+        gpp::DevProblem &dp = b->dps[i];
+        dp.idx = static_cast<const int32_t *>(o[0].ptr);
+        dp.wx = static_cast<const double *>(o[1].ptr);
+        dp.vcoul = static_cast<const double *>(o[2].ptr);
+        dp.aqsm = static_cast<const cplx *>(o[3].ptr);
+        dp.aqsn = static_cast<const cplx *>(o[4].ptr);
+        dp.ieps = static_cast<const cplx *>(o[5].ptr);
+        dp.wtilde = static_cast<const cplx *>(o[6].ptr);
+    }
+    *out = b;
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
+{
+    if (!b || !outs) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (form != GPP_FORM_FACTORED_F64) return fail(GPP_ERR_UNSUPPORTED, "batches run GPP_FORM_FACTORED_F64 only");
+    for (int i = 0; i < b->n; ++i) {
+        if (!outs[i].achtemp || !outs[i].asxtemp)
+            return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: achtemp and asxtemp are required", i);
+        if (outs[i].acht_n1_loc)
+            return fail(GPP_ERR_UNSUPPORTED, "instance %d: acht_n1_loc comes from the direct forms only", i);
+    }
+    gpp_status st = set_device(b->ctx);
+    if (st) return st;
+    hipStream_t s = b->ctx->stream;
+    if (!b->planned && (st = batch_prepare(b, b->kind))) return st;
+    if (b->use_graph && !b->exec && (st = batch_capture(b))) return st;
+    HIP_OR_FAIL(hipEventRecord(b->ev0, s), "hipEventRecord");
+    if (b->use_graph) {
+        HIP_OR_FAIL(hipGraphLaunch(b->exec, s), "hipGraphLaunch");
+    } else {  // stage by stage, with events between the stages
+        for (int st_i = 0; st_i < 4; ++st_i) {
+            HIP_OR_FAIL(gpp::launch_batch(b->plan, b->work.ptr, batch_blocks(b), s, 1 << st_i), "batch launch");
+            if (st_i < 3) HIP_OR_FAIL(hipEventRecord(b->evs[st_i], s), "hipEventRecord");
+        }
+    }
+    HIP_OR_FAIL(hipEventRecord(b->ev1, s), "hipEventRecord");
+    std::vector<std::vector<cplx>> host(b->n);
+    for (int i = 0; i < b->n; ++i) {
+        const gpp::DevProblem &dp = b->dps[i];
+        host[i].resize((size_t)2 * dp.nfreq + dp.nb + 1);
+        const char *src = static_cast<const char *>(b->work.ptr) + gpp::batch_results_offset(b->plan, i);
+        HIP_OR_FAIL(hipMemcpyAsync(host[i].data(), src, sizeof(cplx) * host[i].size(), hipMemcpyDeviceToHost, s),
+                    "D2H batch results");
+    }
+    HIP_OR_FAIL(hipStreamSynchronize(s), "batch synchronize");
+    float ms = 0.f;
+    HIP_OR_FAIL(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
+    b->last_ms = ms;
+    for (int q = 0; q < 4; ++q) b->stage_ms[q] = 0.0;
+    if (!b->use_graph) {
+        hipEvent_t marks[5] = {b->ev0, b->evs[0], b->evs[1], b->evs[2], b->ev1};
+        for (int q = 0; q < 4; ++q) {
+            HIP_OR_FAIL(hipEventElapsedTime(&ms, marks[q], marks[q + 1]), "hipEventElapsedTime");
+            b->stage_ms[q] = ms;
+        }
+    }
+    for (int i = 0; i < b->n; ++i) {
+        const gpp::DevProblem &dp = b->dps[i];
+        memcpy(outs[i].achtemp, host[i].data(), sizeof(cplx) * dp.nfreq);
+        memcpy(outs[i].asxtemp, host[i].data() + dp.nfreq, sizeof(cplx) * dp.nfreq);
+        if (outs[i].achstemp) memcpy(outs[i].achstemp, host[i].data() + 2 * dp.nfreq + dp.nb, sizeof(cplx));
+    }
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_autotune(gpp_batch *b, gpp_form form)
+{
+    if (!b) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL batch");
+    if (form != GPP_FORM_FACTORED_F64) return fail(GPP_ERR_UNSUPPORTED, "batches run GPP_FORM_FACTORED_F64 only");
+    gpp_status st = set_device(b->ctx);
+    if (st) return st;
+    hipStream_t s = b->ctx->stream;
+    // time the stages selected by `which` for kind k (best of two after a warm-up)
+    auto timed = [&](const gpp::GramKind &k, int which, double *ms_out) -> gpp_status {
+        gpp_status st2 = batch_prepare(b, k);
+        if (st2) return st2;
+        double best = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_OR_FAIL(gpp::launch_batch(b->plan, b->work.ptr, batch_blocks(b), s, 15 & ~which), "autotune launch");
+            HIP_OR_FAIL(hipEventRecord(b->ev0, s), "hipEventRecord");
+            HIP_OR_FAIL(gpp::launch_batch(b->plan, b->work.ptr, batch_blocks(b), s, which), "autotune launch");
+            HIP_OR_FAIL(hipEventRecord(b->ev1, s), "hipEventRecord");
+            HIP_OR_FAIL(hipStreamSynchronize(s), "autotune synchronize");
+            float ms = 0.f;
+            HIP_OR_FAIL(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
+            if (rep) best = std::min(best, (double)ms);
+        }
+        *ms_out = best;
+        return GPP_OK;
+    };
+    gpp::GramKind k = b->kind, best_k = b->kind;
+    double best = 1e300, ms = 0;
+    for (int tile = 0; tile < gpp::kGramTiles; ++tile) {
+        k.tile = tile;
+        if ((st = timed(k, 2, &ms))) return st;
+        if (ms < best) best = ms, best_k.tile = tile;
+    }
+    k = best_k;
+    best = 1e300;
+    const int chunks[] = {2048, 4096, 8192, 16384};
+    for (int chunk : chunks)
+        for (int fc : {4, 8}) {
+            k.red_chunk = chunk;
+            k.red_fc = fc;
+            if ((st = timed(k, 4, &ms))) return st;
+            if (ms < best) best = ms, best_k.red_chunk = chunk, best_k.red_fc = fc;
+        }
+    return batch_prepare(b, best_k);
+}
+
+gpp_status gpp_batch_use_graph(gpp_batch *b, int use_graph)
+{
+    if (!b) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL batch");
+    b->use_graph = use_graph != 0;
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_last_timing(const gpp_batch *b, double *ms, int *gram_tile, int *reduce_chunk, int *reduce_freqs)
+{
+    if (!b) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL batch");
+    if (ms) *ms = b->last_ms;
+    if (gram_tile) *gram_tile = b->kind.tile;
+    if (reduce_chunk) *reduce_chunk = b->kind.red_chunk;
+    if (reduce_freqs) *reduce_freqs = b->kind.red_fc;
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_last_stages(const gpp_batch *b, double *stage_ms)
+{
+    if (!b || !stage_ms) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int q = 0; q < 4; ++q) stage_ms[q] = b->stage_ms[q];
+    return GPP_OK;
+}
+
+gpp_status gpp_batch_destroy(gpp_batch *b)
+{
+    if (!b) return GPP_OK;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    batch_drop_graph(b);
+    for (DevBuf &d : b->owned) d.release();
+    b->work.release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipEvent_t e : b->evs)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+    return GPP_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,252 @@
+// gpp_batch.hip -- many independent GPP sigma problems of uneven size in one
+// pass of the factored FP64 form.
+//
+// Each stage of the factored form (gpp_factored_dev.hpp) becomes ONE
+// persistent launch over a work list that spans every instance:
+//   gather    flat element index over all instances' Bg[nb][ngpown]
+//   achstemp  (instance, band) items
+//   Gram      (instance, G' tile, G tile) items of the FP64-MFMA product
+//   reduce    (instance, column, G-vector split, frequency chunk) items
+//   final     one block per instance
+// Lists are built on the host, sorted by decreasing cost, and walked with a
+// static round-robin (block b takes items b, b+G, b+2G, ...), so every item is
+// computed by exactly one block and results do not depend on scheduling.
+// The five launches are captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <vector>
+
+#include "gpp_factored_dev.hpp"
+
+namespace gpp {
+namespace {
+
+using namespace fact;
+
+struct BInst {
+    const cplx *aqsm, *aqsn, *ieps, *wtilde;
+    const int32_t *idx;
+    const double *vcoul, *wx;
+    int nb, nv, ncouls, ngpown, nfreq, nsplit;
+    double occ;
+    cplx *bg, *mu, *col_ach, *col_asx, *part_achs, *out;
+};
+
+__device__ __forceinline__ int find_inst(const size_t *off, int n, size_t t)
+{
+    int lo = 0, hi = n - 1;  // largest i with off[i] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_batch_gather(const BInst *__restrict__ insts, int n, const size_t *__restrict__ off)
+{
+    const size_t total = off[n];
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = find_inst(off, n, t);
+        const BInst &b = insts[i];
+        gather_elem(b.aqsm, b.idx, b.ncouls, b.ngpown, t - off[i], b.bg);
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_batch_achstemp(const BInst *__restrict__ insts,
+                                                                const int2 *__restrict__ items, int nitems)
+{
+    __shared__ cplx scratch[kRedThreads / kWave];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const int2 it = items[w];
+        const BInst &b = insts[it.x];
+        const cplx s = achstemp_band(b.aqsm, b.aqsn, b.ieps, b.idx, b.vcoul, b.ncouls, b.ngpown, it.y, scratch);
+        if (threadIdx.x == 0) b.part_achs[it.y] = s;
+    }
+}
+
+template <int WJN, int WIN>
+__global__ __launch_bounds__(64 * WJN * WIN) void k_batch_gram(const BInst *__restrict__ insts,
+                                                               const GramItem *__restrict__ items, int nitems)
+{
+    typedef GramCfg<WJN, WIN> C;
+    __shared__ cplx sB[2][kKS][C::TJ];
+    __shared__ cplx sA[2][kKS][C::TI];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const GramItem it = items[w];
+        const BInst &b = insts[it.inst];
+        gram_f64_tile<WJN, WIN>(b.bg, b.aqsn, b.ngpown, b.ncouls, b.nv, b.nb, it.j0, it.i0, sB, sA, b.mu);
+    }
+}
+
+template <int FC>
+__global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__restrict__ insts,
+                                                              const RedItem *__restrict__ items, int nitems,
+                                                              int chunk)
+{
+    __shared__ cplx scratch[kRedThreads / kWave];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const RedItem it = items[w];
+        const BInst &b = insts[it.inst];
+        ReduceArgs ra;
+        ra.mu = b.mu; ra.ieps = b.ieps; ra.wtilde = b.wtilde; ra.aqsn = b.aqsn; ra.bg = b.bg; ra.idx = b.idx;
+        ra.vcoul = b.vcoul; ra.wx = b.wx; ra.nv = b.nv; ra.ncouls = b.ncouls; ra.ngpown = b.ngpown;
+        ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
+        ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
+        pole_reduce_item<FC>(ra, it.j, it.split, it.iw0, scratch);
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_batch_final(const BInst *__restrict__ insts)
+{
+    __shared__ cplx scratch[kRedThreads / kWave];
+    const BInst &b = insts[blockIdx.x];
+    final_block(b.col_ach, b.col_asx, b.part_achs, b.ngpown * b.nsplit, b.nfreq, b.nb, b.nv, b.out, scratch);
+}
+
+__host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr int gram_tj(int tile) { return tile == 0 || tile == 2 ? 128 : 64; }
+constexpr int gram_ti(int tile) { return tile == 0 || tile == 1 ? 128 : 64; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes)
+{
+    BatchPlan pl;
+    pl.n = (int)probs.size();
+    pl.kind = k;
+    size_t off = 0;
+    pl.inst_off.resize(pl.n);
+    for (int i = 0; i < pl.n; ++i) {
+        const DevProblem &p = probs[i];
+        const int nsplit = (p.ncouls + k.red_chunk - 1) / k.red_chunk;
+        BatchPlan::InstOff o;
+        o.nsplit = nsplit;
+        o.bg = off;        off += align256(sizeof(cplx) * (size_t)p.nb * p.ngpown);
+        o.mu = off;        off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
+        o.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
+        o.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
+        o.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
+        o.out = off;       off += align256(sizeof(cplx) * ((size_t)2 * p.nfreq + p.nb + 1));
+        pl.inst_off[i] = o;
+    }
+    // device tables after the per-instance areas
+    pl.tab_inst = off;  off += align256(sizeof(BInst) * pl.n);
+    pl.tab_goff = off;  off += align256(sizeof(size_t) * (pl.n + 1));
+    // work lists
+    std::vector<std::pair<long, GramItem>> g;
+    std::vector<std::pair<long, RedItem>> r;
+    std::vector<int2> a;
+    const int tj = gram_tj(k.tile), ti = gram_ti(k.tile);
+    for (int i = 0; i < pl.n; ++i) {
+        const DevProblem &p = probs[i];
+        const long K = p.nb - p.nv;
+        for (int i0 = 0; i0 < p.ncouls; i0 += ti)
+            for (int j0 = 0; j0 < p.ngpown; j0 += tj) g.push_back({K, GramItem{i, j0, i0, 0}});
+        const int fc = k.red_fc == 4 ? 4 : 8;  // one FC for the whole launch; extra slots are masked
+        const int nsplit = pl.inst_off[i].nsplit;
+        for (int iw0 = 0; iw0 < p.nfreq; iw0 += fc)
+            for (int j = 0; j < p.ngpown; ++j)
+                for (int sp = 0; sp < nsplit; ++sp) {
+                    const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
+                    r.push_back({len * fc * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp, iw0}});
+                }
+        for (int n1 = 0; n1 < p.nb; ++n1) a.push_back(int2{i, n1});
+    }
+    auto by_cost = [](const auto &x, const auto &y) { return x.first > y.first; };
+    std::stable_sort(g.begin(), g.end(), by_cost);
+    std::stable_sort(r.begin(), r.end(), by_cost);
+    pl.gram_items.clear();
+    for (auto &e : g) pl.gram_items.push_back(e.second);
+    for (auto &e : r) pl.red_items.push_back(e.second);
+    pl.ach_items = a;
+    pl.tab_gram = off; off += align256(sizeof(GramItem) * (pl.gram_items.size() + 1));
+    pl.tab_red = off;  off += align256(sizeof(RedItem) * (pl.red_items.size() + 1));
+    pl.tab_ach = off;  off += align256(sizeof(int2) * (pl.ach_items.size() + 1));
+    pl.total = off;
+    pl.fc = k.red_fc == 4 ? 4 : 8;
+    if (work_bytes) *work_bytes = pl.total;
+    return pl;
+}
+
+hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &probs, void *work, hipStream_t s)
+{
+    char *w = static_cast<char *>(work);
+    std::vector<BInst> tab(pl.n);
+    std::vector<size_t> goff(pl.n + 1, 0);
+    for (int i = 0; i < pl.n; ++i) {
+        const DevProblem &p = probs[i];
+        const BatchPlan::InstOff &o = pl.inst_off[i];
+        BInst &b = tab[i];
+        b.aqsm = p.aqsm; b.aqsn = p.aqsn; b.ieps = p.ieps; b.wtilde = p.wtilde; b.idx = p.idx;
+        b.vcoul = p.vcoul; b.wx = p.wx;
+        b.nb = p.nb; b.nv = p.nv; b.ncouls = p.ncouls; b.ngpown = p.ngpown; b.nfreq = p.nfreq;
+        b.nsplit = o.nsplit; b.occ = p.occ;
+        b.bg = reinterpret_cast<cplx *>(w + o.bg);
+        b.mu = reinterpret_cast<cplx *>(w + o.mu);
+        b.col_ach = reinterpret_cast<cplx *>(w + o.col_ach);
+        b.col_asx = reinterpret_cast<cplx *>(w + o.col_asx);
+        b.part_achs = reinterpret_cast<cplx *>(w + o.part_achs);
+        b.out = reinterpret_cast<cplx *>(w + o.out);
+        goff[i + 1] = goff[i] + (size_t)p.nb * p.ngpown;
+    }
+    hipError_t e;
+    if ((e = hipMemcpyAsync(w + pl.tab_inst, tab.data(), sizeof(BInst) * pl.n, hipMemcpyHostToDevice, s))) return e;
+    if ((e = hipMemcpyAsync(w + pl.tab_goff, goff.data(), sizeof(size_t) * (pl.n + 1), hipMemcpyHostToDevice, s)))
+        return e;
+    if (!pl.gram_items.empty() &&
+        (e = hipMemcpyAsync(w + pl.tab_gram, pl.gram_items.data(), sizeof(GramItem) * pl.gram_items.size(),
+                            hipMemcpyHostToDevice, s)))
+        return e;
+    if (!pl.red_items.empty() &&
+        (e = hipMemcpyAsync(w + pl.tab_red, pl.red_items.data(), sizeof(RedItem) * pl.red_items.size(),
+                            hipMemcpyHostToDevice, s)))
+        return e;
+    if (!pl.ach_items.empty() &&
+        (e = hipMemcpyAsync(w + pl.tab_ach, pl.ach_items.data(), sizeof(int2) * pl.ach_items.size(),
+                            hipMemcpyHostToDevice, s)))
+        return e;
+    return hipStreamSynchronize(s);
+}
+
+// Launches the batch's stages; `which` selects stages (bit 0 gather+achstemp,
+// bit 1 Gram, bit 2 reduce, bit 3 final) so the autotuner can time one stage.
+hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, hipStream_t s, int which)
+{
+    char *w = static_cast<char *>(work);
+    const BInst *insts = reinterpret_cast<const BInst *>(w + pl.tab_inst);
+    const size_t *goff = reinterpret_cast<const size_t *>(w + pl.tab_goff);
+    const int G = persistent_blocks;
+    if (which & 1) {
+        hipLaunchKernelGGL(k_batch_gather, dim3(G), dim3(256), 0, s, insts, pl.n, goff);
+        hipLaunchKernelGGL(k_batch_achstemp, dim3(std::min<size_t>(G, pl.ach_items.size() + 1)), dim3(kRedThreads), 0,
+                           s, insts, reinterpret_cast<const int2 *>(w + pl.tab_ach), (int)pl.ach_items.size());
+    }
+    if (which & 2) {
+        const GramItem *gi = reinterpret_cast<const GramItem *>(w + pl.tab_gram);
+        const int ng = (int)pl.gram_items.size();
+        const dim3 grid(std::max(1, std::min(G, ng)));
+        switch (pl.kind.tile) {
+        case 0: hipLaunchKernelGGL((k_batch_gram<2, 4>), grid, dim3(512), 0, s, insts, gi, ng); break;
+        case 1: hipLaunchKernelGGL((k_batch_gram<1, 4>), grid, dim3(256), 0, s, insts, gi, ng); break;
+        case 2: hipLaunchKernelGGL((k_batch_gram<2, 2>), grid, dim3(256), 0, s, insts, gi, ng); break;
+        default: hipLaunchKernelGGL((k_batch_gram<1, 2>), grid, dim3(128), 0, s, insts, gi, ng); break;
+        }
+    }
+    if (which & 4) {
+        const RedItem *ri = reinterpret_cast<const RedItem *>(w + pl.tab_red);
+        const int nr = (int)pl.red_items.size();
+        const dim3 grid(std::max(1, std::min(G, nr)));
+        if (pl.fc == 4)
+            hipLaunchKernelGGL((k_batch_reduce<4>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, pl.kind.red_chunk);
+        else
+            hipLaunchKernelGGL((k_batch_reduce<8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, pl.kind.red_chunk);
+    }
+    if (which & 8) hipLaunchKernelGGL(k_batch_final, dim3(pl.n), dim3(kRedThreads), 0, s, insts);
+    return hipGetLastError();
+}
+
+size_t batch_results_offset(const BatchPlan &pl, int i) { return pl.inst_off[i].out; }
+
+}  // namespace gpp

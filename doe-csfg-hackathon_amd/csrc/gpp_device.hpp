@@ -143,6 +143,13 @@ __device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rw
     }
 }
 
+// reduce_achstemp (gppComplex.cpp:66-101) for band n1, summed over the
+// columns in a fixed block order (block_sum; every thread gets the total).
+// The else branch of :94-98 is unreachable once negative indices are refused.
+__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
+                                               const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
+                                               int n1, cplx *scratch);
+
 // igp = min(inv_igp_index[j], ncouls-1), gppComplex.cpp:342-344.  Negative
 // entries are rejected before any launch; the max(.,0) only keeps a kernel
 // in bounds should one slip through.
@@ -151,6 +158,24 @@ __device__ __forceinline__ int igp_of(const int32_t *idx, int j, int ncouls)
     int igp = idx[j];
     igp = igp >= ncouls ? ncouls - 1 : igp;
     return igp < 0 ? 0 : igp;
+}
+
+__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
+                                               const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
+                                               int n1, cplx *scratch)
+{
+    cplx s = czero();
+    for (int j = threadIdx.x; j < ngpown; j += blockDim.x) {
+        const int igp = igp_of(idx, j, ncouls);
+        const cplx mygpvar1 = cconj(aqsm[(size_t)n1 * ncouls + igp]);           // :84
+        const cplx ie = ieps[(size_t)j * ncouls + igp];
+        const cplx schs = cplx{-ie.re, -ie.im};                                  // :86-88
+        const cplx matngmatmgp = cmul(aqsn[(size_t)n1 * ncouls + igp], mygpvar1); // :89
+        cplx schstemp = czero();
+        if (cabs_(schs) > kTo1) cacc(schstemp, cmul(matngmatmgp, schs));          // :91-92
+        cacc(s, cscale(schstemp, vcoul[igp] * 0.5));                              // :99
+    }
+    return block_sum(s, scratch);
 }
 
 }  // namespace gpp

@@ -332,20 +332,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_achstemp(DirectArgs a)
 {
     __shared__ cplx scratch[kReduceThreads / kWave];
     const int n1 = blockIdx.x;
-    const int ncouls = a.ncouls;
-    cplx s = czero();
-    for (int j = threadIdx.x; j < a.ngpown; j += blockDim.x) {
-#pragma clang fp contract(off)
-        const int igp = igp_of(a.idx, j, ncouls);
-        const cplx mygpvar1 = cconj(a.aqsm[(size_t)n1 * ncouls + igp]);        // :84
-        const cplx ie = a.ieps[(size_t)j * ncouls + igp];
-        const cplx schs = cplx{-ie.re, -ie.im};                                 // :86-88
-        const cplx matngmatmgp = cmul(a.aqsn[(size_t)n1 * ncouls + igp], mygpvar1);  // :89
-        cplx schstemp = czero();
-        if (cabs_(schs) > kTo1) cacc(schstemp, cmul(matngmatmgp, schs));         // :91-92
-        cacc(s, cscale(schstemp, a.vcoul[igp] * 0.5));                           // :99
-    }
-    s = block_sum(s, scratch);
+    const cplx s = achstemp_band(a.aqsm, a.aqsn, a.ieps, a.idx, a.vcoul, a.ncouls, a.ngpown, n1, scratch);
     if (threadIdx.x == 0) a.part_achs[n1] = s;
 }
 

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "gpp_device.hpp"
 
 namespace gpp {
@@ -89,12 +91,16 @@ hipError_t launch_accumulate_compat(const DevResults &res, int which, int nv, do
 hipError_t launch_fill_seeded(cplx *dst, size_t n, uint64_t seed, uint32_t array_id, double re_lo,
                               double re_hi, double im_lo, double im_hi, hipStream_t s);
 
-// How the factored form computes its Gram product.
+// How the factored form computes its Gram product and reduce.
 struct GramKind {
     int ozaki_slices = 0;  // 0: FP64 MFMA; 3..6: Ozaki split with that many slices
     bool int8 = false;     // Ozaki digits in INT8 (else FP16)
     int layout = 1;        // Ozaki wave layout (0: 2x2 waves of 32x32, 1: 2x4 waves of 32x16)
+    int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
+    int red_chunk = 8192;  // G-vectors per pole-reduce block
+    int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
 };
+constexpr int kGramTiles = 4;
 
 size_t factored_workspace_bytes(const DevProblem &p, GramKind kind);
 hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const DevResults &res,
@@ -105,5 +111,38 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
 size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8);
 hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, const cplx *bg, void *work,
                              cplx *mout, hipStream_t s);
+
+// ---- batched factored form (gpp_batch.hip) --------------------------------
+struct GramItem {  // one Gram tile of one instance
+    int inst, j0, i0, pad;
+};
+struct RedItem {   // one pole-reduce item of one instance
+    int inst, j, split, iw0;
+};
+
+// Workspace layout and work lists of one batch for one GramKind.
+struct BatchPlan {
+    struct InstOff {
+        size_t bg, mu, col_ach, col_asx, part_achs, out;
+        int nsplit;
+    };
+    int n = 0;
+    int fc = 8;
+    GramKind kind;
+    std::vector<InstOff> inst_off;
+    std::vector<GramItem> gram_items;
+    std::vector<RedItem> red_items;
+    std::vector<int2> ach_items;
+    size_t tab_inst = 0, tab_goff = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
+};
+
+BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes);
+// Writes the device tables of the plan into `work` (synchronous on s).
+hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &probs, void *work, hipStream_t s);
+// Enqueues the batch's persistent stages; which: bit 0 gather+achstemp, bit 1
+// Gram, bit 2 reduce, bit 3 final (15 = all).
+hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, hipStream_t s, int which);
+// Offset in `work` of instance i's results (DevResults layout).
+size_t batch_results_offset(const BatchPlan &pl, int i);
 
 }  // namespace gpp

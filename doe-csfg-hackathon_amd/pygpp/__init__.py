@@ -35,6 +35,9 @@ GPP_ERR_NO_INPUTS = 7
 # gpp_option
 GPP_OPT_OZAKI_SLICES = 1
 GPP_OPT_OZAKI_LAYOUT = 2
+GPP_OPT_GRAM_TILE = 3
+GPP_OPT_REDUCE_CHUNK = 4
+GPP_OPT_REDUCE_FREQS = 5
 
 # gpp_form
 FORMS = {
@@ -61,6 +64,15 @@ EXPORTED_SYMBOLS = (
     "gpp_ctx_get_option",
     "gpp_ctx_last_timing",
     "gpp_ctx_stage_name",
+    "gpp_ctx_autotune",
+    "gpp_batch_create",
+    "gpp_batch_create_seeded",
+    "gpp_batch_sigma",
+    "gpp_batch_autotune",
+    "gpp_batch_use_graph",
+    "gpp_batch_last_timing",
+    "gpp_batch_last_stages",
+    "gpp_batch_destroy",
     "d_till_nvband",
     "d_reduce_achstemp",
     "d_noFlagOCCSolver",
@@ -85,6 +97,11 @@ class GppOutputs(ctypes.Structure):
         ("acht_n1_loc", ctypes.c_void_p),
         ("achstemp", ctypes.c_void_p),
     ]
+
+
+class GppDeviceInputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("inv_igp_index", "wx_array", "vcoul", "aqsmtemp", "aqsntemp",
+                                              "I_eps_array", "wtilde_array")]
 
 
 class GppError(RuntimeError):
@@ -123,12 +140,24 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_ctx_get_option.argtypes = [vp, i32, P(i32)]
     lib.gpp_ctx_stage_name.restype = ctypes.c_char_p
     lib.gpp_ctx_stage_name.argtypes = [vp, i32]
+    lib.gpp_ctx_autotune.argtypes = [vp, i32]
+    lib.gpp_batch_create.argtypes = [vp, i32, P(GppProblem), P(GppDeviceInputs), P(vp)]
+    lib.gpp_batch_create_seeded.argtypes = [vp, i32, P(GppProblem), P(ctypes.c_uint64), P(dbl), P(dbl), P(vp)]
+    lib.gpp_batch_sigma.argtypes = [vp, i32, P(GppOutputs)]
+    lib.gpp_batch_autotune.argtypes = [vp, i32]
+    lib.gpp_batch_use_graph.argtypes = [vp, i32]
+    lib.gpp_batch_last_timing.argtypes = [vp, P(dbl), P(i32), P(i32), P(i32)]
+    lib.gpp_batch_last_stages.argtypes = [vp, P(dbl)]
+    lib.gpp_batch_destroy.argtypes = [vp]
     lib.d_till_nvband.argtypes = [i32, i32, i32] + [vp] * 11 + [dbl]
     lib.d_reduce_achstemp.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, i32, vp]
     lib.d_noFlagOCCSolver.argtypes = [i32, i32, i32, i32] + [vp] * 10
     for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device",
                  "gpp_fill_deterministic", "gpp_fill_seeded", "gpp_generate_seeded", "gpp_sigma",
-                 "gpp_ctx_last_timing", "gpp_ctx_set_option", "gpp_ctx_get_option", "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
+                 "gpp_ctx_last_timing", "gpp_ctx_set_option", "gpp_ctx_get_option", "gpp_ctx_autotune",
+                 "gpp_batch_create", "gpp_batch_create_seeded", "gpp_batch_sigma", "gpp_batch_autotune",
+                 "gpp_batch_use_graph", "gpp_batch_last_timing", "gpp_batch_last_stages", "gpp_batch_destroy",
+                 "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
         getattr(lib, name).restype = i32
     _lib = lib
     return lib
@@ -267,6 +296,13 @@ class Context:
         out["achstemp"] = out["achstemp"][0]
         return out
 
+    def autotune(self, form: str = "factored-f64") -> Dict[str, int]:
+        """Times candidate Gram tiles / reduce schedules and keeps the fastest."""
+        _check(load().gpp_ctx_autotune(self._h, FORMS[form]))
+        return {"gram_tile": self.get_option(GPP_OPT_GRAM_TILE),
+                "reduce_chunk": self.get_option(GPP_OPT_REDUCE_CHUNK),
+                "reduce_freqs": self.get_option(GPP_OPT_REDUCE_FREQS)}
+
     def set_option(self, option: int, value: int) -> None:
         _check(load().gpp_ctx_set_option(self._h, option, value))
 
@@ -286,3 +322,76 @@ class Context:
         for i in range(n.value):
             res[lib.gpp_ctx_stage_name(self._h, i).decode()] = stages[i]
         return res
+
+
+class Batch:
+    """A gpp_batch: many problems through the persistent factored pipeline.
+
+    Either ``seeds`` (library-owned inputs generated on the device; optional
+    per-instance ``wx_center`` and ``dw``) or ``device_inputs`` (a list of
+    dicts of device addresses, keys as in gpp_device_inputs) must be given.
+    """
+
+    def __init__(self, ctx: Context, problems, seeds=None, wx_center=None, dw=None, device_inputs=None):
+        lib = load()
+        n = len(problems)
+        self.problems = list(problems)
+        self._ctx = ctx  # keeps the context alive
+        probs = (GppProblem * n)(*[p.c() for p in problems])
+        h = ctypes.c_void_p()
+        if device_inputs is not None:
+            ins = (GppDeviceInputs * n)(*[GppDeviceInputs(*[d[k] for k, _ in GppDeviceInputs._fields_])
+                                          for d in device_inputs])
+            _check(lib.gpp_batch_create(ctx._h, n, probs, ins, ctypes.byref(h)))
+        else:
+            sd = (ctypes.c_uint64 * n)(*seeds)
+            wc = (ctypes.c_double * n)(*wx_center) if wx_center is not None else None
+            dd = (ctypes.c_double * n)(*dw) if dw is not None else None
+            _check(lib.gpp_batch_create_seeded(ctx._h, n, probs, sd, wc, dd, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            load().gpp_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sigma(self, form: str = "factored-f64"):
+        outs_np = []
+        outs = (GppOutputs * len(self.problems))()
+        for i, p in enumerate(self.problems):
+            o = {"achtemp": np.zeros(p.nfreq, np.complex128), "asxtemp": np.zeros(p.nfreq, np.complex128),
+                 "achstemp": np.zeros(1, np.complex128)}
+            outs[i] = GppOutputs(_ptr(o["achtemp"]), _ptr(o["asxtemp"]), None, _ptr(o["achstemp"]))
+            outs_np.append(o)
+        _check(load().gpp_batch_sigma(self._h, FORMS[form], outs))
+        for o in outs_np:
+            o["achstemp"] = o["achstemp"][0]
+        return outs_np
+
+    def autotune(self, form: str = "factored-f64") -> None:
+        _check(load().gpp_batch_autotune(self._h, FORMS[form]))
+
+    def use_graph(self, flag: bool) -> None:
+        _check(load().gpp_batch_use_graph(self._h, int(bool(flag))))
+
+    def last_timing(self) -> Dict[str, float]:
+        ms = ctypes.c_double()
+        t, c, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load().gpp_batch_last_timing(self._h, ctypes.byref(ms), ctypes.byref(t), ctypes.byref(c),
+                                            ctypes.byref(f)))
+        st = (ctypes.c_double * 4)()
+        _check(load().gpp_batch_last_stages(self._h, st))
+        return {"ms": ms.value, "gram_tile": t.value, "reduce_chunk": c.value, "reduce_freqs": f.value,
+                "stages_ms": {"gather_achstemp": st[0], "gram_f64": st[1], "pole_reduce": st[2], "final": st[3]}}

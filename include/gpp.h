@@ -104,7 +104,15 @@ typedef enum {
     /* Wave layout of the Ozaki Gram kernel's 64x64 block tile: 0 = 2x2 waves
      * of 32x32 (one wave per SIMD), 1 = 2x4 waves of 32x16 (two per SIMD,
      * default). */
-    GPP_OPT_OZAKI_LAYOUT = 2
+    GPP_OPT_OZAKI_LAYOUT = 2,
+    /* FP64-MFMA Gram tile (G' columns x G-vectors): 0 = 128x128 (default),
+     * 1 = 64x128, 2 = 128x64, 3 = 64x64. */
+    GPP_OPT_GRAM_TILE = 3,
+    /* G-vectors per pole-reduce block: a power of two in [256, 2^20]
+     * (default 8192). */
+    GPP_OPT_REDUCE_CHUNK = 4,
+    /* Frequencies per pole-reduce pass when nfreq > 4: 4 or 8 (default 8). */
+    GPP_OPT_REDUCE_FREQS = 5
 } gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).
@@ -219,6 +227,69 @@ gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *sta
 
 /* Name of stage i of the last gpp_sigma ("" when out of range). */
 const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i);
+
+/*
+ * Times the candidate Gram tiles and pole-reduce schedules of a factored
+ * form on the inputs last uploaded or bound, and sets the fastest as the
+ * context's GPP_OPT_GRAM_TILE / GPP_OPT_REDUCE_CHUNK / GPP_OPT_REDUCE_FREQS.
+ * Results are cached per problem shape in the context.  (GPP_FORM_DIRECT
+ * forms have nothing to tune and return GPP_OK.)
+ */
+gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form);
+
+/* ---- batches: many independent problems in one pass ------------------------- */
+/*
+ * A batch evaluates n independent problems of different sizes with the
+ * factored FP64 form: every stage is one persistent launch over a work list
+ * spanning all instances, and the launches are captured once into a hipGraph
+ * that each gpp_batch_sigma replays.  The batch uses its context's device
+ * and stream; destroy it before the context.
+ */
+typedef struct gpp_batch gpp_batch;
+
+/* Device arrays of one problem (same layout as gpp_bind_device). */
+typedef struct {
+    const int32_t *inv_igp_index;
+    const double *wx_array;
+    const double *vcoul;
+    const gpp_complex *aqsmtemp;
+    const gpp_complex *aqsntemp;
+    const gpp_complex *I_eps_array;
+    const gpp_complex *wtilde_array;
+} gpp_device_inputs;
+
+/* Batch over caller-owned device inputs (kept alive and unchanged). */
+gpp_status gpp_batch_create(gpp_ctx *ctx, int n, const gpp_problem *probs, const gpp_device_inputs *inputs,
+                            gpp_batch **out);
+
+/* Batch over library-owned inputs generated on the device like
+ * gpp_generate_seeded, instance i with seeds[i] and the frequency grid
+ * wx_center[i] + dw[i]*(iw - (nfreq-1)/2) (NULL arrays: 4.0 and 1.0). */
+gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs, const uint64_t *seeds,
+                                   const double *wx_center, const double *dw, gpp_batch **out);
+
+/* Evaluates every instance; outs[i] receives instance i (acht_n1_loc must be
+ * NULL).  Only GPP_FORM_FACTORED_F64 is batched. */
+gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs);
+
+/* Times the Gram tiles and reduce schedules on this batch and keeps the
+ * fastest (re-planning and re-capturing the graph). */
+gpp_status gpp_batch_autotune(gpp_batch *b, gpp_form form);
+
+/* 1 (default): replay the captured hipGraph; 0: launch the same kernels eagerly. */
+gpp_status gpp_batch_use_graph(gpp_batch *b, int use_graph);
+
+/* GPU milliseconds of the last gpp_batch_sigma (HIP events around the
+ * launches) and the configuration in use (any pointer may be NULL). */
+gpp_status gpp_batch_last_timing(const gpp_batch *b, double *ms, int *gram_tile, int *reduce_chunk,
+                                 int *reduce_freqs);
+
+/* Per-stage GPU milliseconds of the last gpp_batch_sigma run without the
+ * graph (gpp_batch_use_graph(b, 0)): [gather + achstemp, Gram product,
+ * pole reduce, final]; zeros after a graph replay. */
+gpp_status gpp_batch_last_stages(const gpp_batch *b, double *stage_ms);
+
+gpp_status gpp_batch_destroy(gpp_batch *b);
 
 /* ---- reference-compatible device launchers (GPPComplex.h:306-310) ---------- */
 /*

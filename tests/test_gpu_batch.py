@@ -1,0 +1,123 @@
+"""The batched factored form: many uneven problems through one persistent
+launch per stage, replayed from a captured hipGraph.  Each instance must
+match the oracle (1e-10 per component), and -- for the same tile and reduce
+schedule -- be bitwise identical to the single-problem factored path."""
+import numpy as np
+import pytest
+
+import gpp_inputs
+import hiprt
+from test_gpu_parity import assert_close, problem_of
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+KEYS = ("achtemp", "asxtemp", "achstemp")
+
+SHAPES = [  # nb, nv, ncouls, ngpown, nfreq
+    (24, 2, 700, 35, 3),
+    (9, 0, 130, 9, 1),
+    (40, 4, 1200, 60, 9),
+    (17, 3, 513, 25, 5),
+    (6, 6, 300, 12, 3),
+    (33, 2, 1000, 50, 16),
+]
+
+
+def make_arrays(seed0=100):
+    out = []
+    for i, (nb, nv, nc, ng, nf) in enumerate(SHAPES):
+        a = (gpp_inputs.near_pole_inputs(nb, nc, ng, seed=seed0 + i) if nf == 3 and i % 2 == 0
+             else gpp_inputs.seeded_inputs(nb, nc, ng, nfreq=nf, seed=seed0 + i, dw=2.0 / max(nf - 1, 1)))
+        out.append((a, nv))
+    return out
+
+
+def device_batch(ctx, arrays):
+    import pygpp
+    dev = [{k: hiprt.DeviceArray(v) for k, v in a.items()} for a, _ in arrays]
+    probs = [problem_of(a, nv) for a, nv in arrays]
+    b = pygpp.Batch(ctx, probs, device_inputs=[{k: d.ptr for k, d in dd.items()} for dd in dev])
+    return b, dev
+
+
+def test_batch_vs_oracle(ctx, oracle):
+    arrays = make_arrays()
+    b, dev = device_batch(ctx, arrays)
+    try:
+        outs = b.sigma()
+    finally:
+        b.close()
+    for (a, nv), out in zip(arrays, outs):
+        assert_close(out, oracle.sigma(a, nvband=nv, threads=8), TOL, KEYS)
+
+
+def test_batch_bitwise_equals_single_path(ctx):
+    import pygpp
+    probs = [pygpp.Problem(*s) for s in SHAPES]
+    seeds = [11 + i for i in range(len(probs))]
+    with pygpp.Batch(ctx, probs, seeds=seeds) as b:
+        outs = b.sigma()
+        cfg = b.last_timing()
+    for p, s, out in zip(probs, seeds, outs):
+        ctx.generate_seeded(p, s)
+        ref = ctx.sigma("factored-f64")
+        assert cfg["gram_tile"] == ctx.get_option(pygpp.GPP_OPT_GRAM_TILE)
+        for k in KEYS:
+            np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Replays of the captured graph, with eager work and synchronisation in
+    between, all equal the eager launch bitwise."""
+    import pygpp
+    probs = [pygpp.Problem(*s) for s in SHAPES]
+    with pygpp.Batch(ctx, probs, seeds=list(range(50, 50 + len(probs)))) as b:
+        b.use_graph(False)
+        eager = b.sigma()
+        b.use_graph(True)
+        for rep in range(4):
+            got = b.sigma()
+            # unrelated eager work on the same stream between replays
+            ctx.generate_seeded(pygpp.Problem(8, 2, 256, 16), rep)
+            ctx.sigma("direct")
+            for g, e in zip(got, eager):
+                for k in KEYS:
+                    np.testing.assert_array_equal(g[k], e[k])
+
+
+def test_batch_autotune_keeps_accuracy(ctx, oracle):
+    import pygpp
+    arrays = make_arrays(seed0=300)
+    b, dev = device_batch(ctx, arrays)
+    try:
+        b.autotune()
+        outs = b.sigma()
+        cfg = b.last_timing()
+    finally:
+        b.close()
+    assert 0 <= cfg["gram_tile"] <= 3 and cfg["reduce_freqs"] in (4, 8)
+    for (a, nv), out in zip(arrays, outs):
+        assert_close(out, oracle.sigma(a, nvband=nv, threads=8), TOL, KEYS)
+
+
+def test_batch_rejects_unsupported(ctx):
+    import pygpp
+    with pygpp.Batch(ctx, [pygpp.Problem(8, 2, 200, 10)], seeds=[1]) as b:
+        with pytest.raises(pygpp.GppError) as e:
+            b.sigma("direct")
+        assert e.value.status == pygpp.GPP_ERR_UNSUPPORTED
+
+
+def test_ctx_autotune(ctx, oracle):
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(30, 3000, 200, seed=7)
+    ctx.upload(problem_of(arrs, 2), arrs)
+    try:
+        cfg = ctx.autotune("factored-f64")
+        assert cfg["gram_tile"] in range(4)
+        assert_close(ctx.sigma("factored-f64"), oracle.sigma(arrs, nvband=2, threads=8), TOL, KEYS)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, 8192)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 8)
