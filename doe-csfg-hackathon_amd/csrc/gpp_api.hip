@@ -464,20 +464,18 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (!out || !out->achtemp || !out->asxtemp)
         return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
     if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
-    if ((form == GPP_FORM_FACTORED_F64 || form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) &&
-        out->acht_n1_loc)
-        return fail(GPP_ERR_UNSUPPORTED, "acht_n1_loc is a per-band sum and comes from the direct forms only; "
-                                         "pass NULL with the factored forms");
     gpp_status st = set_device(ctx);
     if (st) return st;
     const gpp::DevProblem &dp = ctx->dp;
     size_t need = 0;
+    gpp::GramKind kind;
     switch (form) {
     case GPP_FORM_DIRECT:
     case GPP_FORM_DIRECT_STRICT: need = gpp::direct_workspace_bytes(dp); break;
     case GPP_FORM_FACTORED_F64:
     case GPP_FORM_FACTORED_OZAKI:
-    case GPP_FORM_FACTORED_OZAKI_I8: need = gpp::factored_workspace_bytes(dp, gram_kind(ctx, form)); break;
+    case GPP_FORM_FACTORED_OZAKI_I8: kind = gram_kind(ctx, form); kind.acht = out->acht_n1_loc != nullptr;
+        need = gpp::factored_workspace_bytes(dp, kind); break;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
     }
     if (need == 0) return fail(GPP_ERR_UNSUPPORTED, "form %d is not available for this problem", (int)form);
@@ -491,7 +489,7 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT)
         e = gpp::run_direct(dp, form == GPP_FORM_DIRECT_STRICT, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     else
-        e = gpp::run_factored(dp, gram_kind(ctx, form), ctx->work.ptr, res, ctx->stream, &ctx->timer);
+        e = gpp::run_factored(dp, kind, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     if (e != hipSuccess) return fail_hip(e, "sigma launch");
     std::vector<cplx> host(res.count());
     HIP_OR_FAIL(hipMemcpyAsync(host.data(), res.base, sizeof(cplx) * res.count(), hipMemcpyDeviceToHost, ctx->stream),

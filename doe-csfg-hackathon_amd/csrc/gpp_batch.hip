@@ -92,6 +92,7 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__res
         ra.vcoul = b.vcoul; ra.wx = b.wx; ra.nv = b.nv; ra.ncouls = b.ncouls; ra.ngpown = b.ngpown;
         ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
+        ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
         pole_reduce_item<FC>(ra, it.j, it.split, it.iw0, scratch);
     }
 }

@@ -10,7 +10,11 @@
 //                      evaluates the pole factors in FP64, reduces per column
 //   4. k_fact_final    fixed-order column reduction; achstemp (reduce_achstemp,
 //                      gppComplex.cpp:66-101) from k_achstemp.
-// acht_n1_loc needs a per-band sum and is not produced by this form.
+// When acht_n1_loc is requested (GramKind::acht), stage 3 also writes the
+// pole factor p2 of frequency 2 (and the occupied branch mask), and
+//   5. k_acht_f64      second FP64 MFMA product Y = R * aqsn^T, split over
+//                      G-vectors, reduced per band in its epilogue
+//   6. k_acht_final    fixed-order sum of the per-tile band partials.
 #include "gpp_factored_dev.hpp"
 
 namespace gpp {
@@ -20,9 +24,34 @@ using namespace fact;
 
 __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// acht product tiles: rows j in 128s for both band ranges; unoccupied bands
+// in columns of 128, occupied bands (few) in columns of 32.
+typedef AchtCfg<2, 4> AchtU;
+typedef AchtCfg<2, 1> AchtO;
+
+struct AchtSplit {
+    int jtiles, nks, chunk;
+};
+
+// Split of the G-vector sum so that the unoccupied launch has ~2048 blocks.
+AchtSplit acht_split(const DevProblem &p)
+{
+    AchtSplit a;
+    a.jtiles = (p.ngpown + AchtU::TJ - 1) / AchtU::TJ;
+    const int ntiles = (p.nb - p.nv + AchtU::TN - 1) / AchtU::TN;
+    const int tiles = a.jtiles * (ntiles > 0 ? ntiles : 1);
+    int nks = (2048 + tiles - 1) / tiles;
+    const int max_nks = p.ncouls / 512 > 1 ? p.ncouls / 512 : 1;
+    if (nks > max_nks) nks = max_nks;
+    a.chunk = ((p.ncouls + nks - 1) / nks + kAK - 1) / kAK * kAK;
+    a.nks = (p.ncouls + a.chunk - 1) / a.chunk;
+    return a;
+}
+
 struct FactLayout {
-    size_t bg, m, col_ach, col_asx, part_achs, total;
+    size_t bg, m, col_ach, col_asx, part_achs, p2, br1, acht_part, total;
     int nsplit;
+    AchtSplit as;
 };
 
 FactLayout fact_layout(const DevProblem &p, const GramKind &k)
@@ -35,6 +64,14 @@ FactLayout fact_layout(const DevProblem &p, const GramKind &k)
     L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
+    L.p2 = L.br1 = L.acht_part = off;
+    L.as = AchtSplit{0, 0, 0};
+    if (k.acht) {
+        L.as = acht_split(p);
+        L.p2 = off;        off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
+        L.br1 = off;       off += p.nv > 0 ? align256((size_t)p.ngpown * p.ncouls) : 0;
+        L.acht_part = off; off += align256(sizeof(cplx) * (size_t)L.as.jtiles * L.as.nks * p.nb);
+    }
     L.total = off;
     return L;
 }
@@ -76,6 +113,39 @@ __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach,
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     final_block(col_ach, col_asx, part_achs, ncols, nfreq, nb, nv, out, scratch);
+}
+
+// Grid (column tiles, j tiles, G-vector splits): the column tiles that share
+// one R tile run together.  part: [jtiles * nks][nb].
+template <int WJN, int WIN, bool MASK>
+__global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restrict__ p2,
+                                                                    const uint8_t *__restrict__ br1,
+                                                                    const cplx *__restrict__ aqsn,
+                                                                    const cplx *__restrict__ bg,
+                                                                    const int32_t *__restrict__ idx,
+                                                                    const double *__restrict__ vcoul, int ngpown,
+                                                                    int ncouls, int nb, int n1_begin, int n1_end,
+                                                                    int chunk, int nks, cplx *__restrict__ part)
+{
+    typedef AchtCfg<WJN, WIN> C;
+    __shared__ cplx sR[2][C::TJ][kAK + 1];
+    __shared__ cplx sA[2][C::TN][kAK + 1];
+    __shared__ cplx red[WJN][C::TN];
+    const int k_begin = blockIdx.z * chunk;
+    const int k_end = k_begin + chunk < ncouls ? k_begin + chunk : ncouls;
+    acht_f64_tile<WJN, WIN, MASK>(p2, br1, aqsn, bg, idx, vcoul, ngpown, ncouls, n1_end, k_begin, k_end,
+                                  blockIdx.y * C::TJ, n1_begin + blockIdx.x * C::TN, sR, sA, red,
+                                  part + ((size_t)blockIdx.y * nks + blockIdx.z) * nb);
+}
+
+// acht_n1_loc[n1] = sum over the (j tile, split) partials in order.
+__global__ void k_acht_final(const cplx *__restrict__ part, int nparts, int nb, cplx *__restrict__ out)
+{
+    const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n1 >= nb) return;
+    cplx s = czero();
+    for (int q = 0; q < nparts; ++q) s = cadd(s, part[(size_t)q * nb + n1]);
+    out[n1] = s;
 }
 
 template <int WJN, int WIN>
@@ -167,12 +237,35 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     ra.vcoul = p.vcoul; ra.wx = p.wx; ra.nv = p.nv; ra.ncouls = p.ncouls; ra.ngpown = p.ngpown;
     ra.nfreq = p.nfreq; ra.occ = p.occ; ra.col_ach = col_ach; ra.col_asx = col_asx;
     ra.nsplit = L.nsplit; ra.chunk = kind.red_chunk;
+    ra.p2 = kind.acht ? reinterpret_cast<cplx *>(w + L.p2) : nullptr;
+    ra.br1 = kind.acht && p.nv > 0 ? reinterpret_cast<uint8_t *>(w + L.br1) : nullptr;
+    ra.acht_iw = p.nfreq > 2 ? 2 : p.nfreq - 1;  // sch_array[2], gppComplex.cpp:381
     GPP_TRY(launch_pole_reduce(ra, kind.red_fc, s));
     if (timer) timer->mark(s, "factored_pole_reduce");
     hipLaunchKernelGGL(k_fact_final, dim3(1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs,
                        p.ngpown * L.nsplit, p.nfreq, p.nb, p.nv, res.base);
     GPP_TRY(hipGetLastError());
     if (timer) timer->mark(s, "factored_final");
+    if (kind.acht) {
+        const AchtSplit &as = L.as;
+        cplx *part = reinterpret_cast<cplx *>(w + L.acht_part);
+        if (p.nb > p.nv) {
+            dim3 grid((p.nb - p.nv + AchtU::TN - 1) / AchtU::TN, as.jtiles, as.nks);
+            hipLaunchKernelGGL((k_acht_f64<2, 4, false>), grid, dim3(AchtU::NT), 0, s, ra.p2, nullptr, p.aqsn, bg,
+                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, part);
+            GPP_TRY(hipGetLastError());
+        }
+        if (p.nv > 0) {
+            dim3 grid((p.nv + AchtO::TN - 1) / AchtO::TN, as.jtiles, as.nks);
+            hipLaunchKernelGGL((k_acht_f64<2, 1, true>), grid, dim3(AchtO::NT), 0, s, ra.p2, ra.br1, p.aqsn, bg,
+                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, 0, p.nv, as.chunk, as.nks, part);
+            GPP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_acht_final, dim3((p.nb + 255) / 256), dim3(256), 0, s, part, as.jtiles * as.nks, p.nb,
+                           res.acht_n1_loc());
+        GPP_TRY(hipGetLastError());
+        if (timer) timer->mark(s, "factored_acht");
+    }
     return hipSuccess;
 }
 

@@ -153,6 +153,13 @@ struct ReduceArgs {
     int nv, ncouls, ngpown, nfreq, nsplit, chunk;
     double occ;
     cplx *col_ach, *col_asx;  // [ngpown][nsplit][nfreq]
+    // Optional (acht_n1_loc): the pole factor of frequency acht_iw, p2 =
+    // delw*I_eps [ngpown][ncouls], and br1 [ngpown][ncouls] = 1 where the
+    // occupied solver takes branch 1 at that frequency (sch = p2 there, 0
+    // elsewhere); p2 null when not requested, br1 null when nv == 0.
+    cplx *p2;
+    uint8_t *br1;
+    int acht_iw;
 };
 
 // Stage 3: pole terms of column j over G-vectors [split*chunk, +chunk), for
@@ -202,10 +209,12 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);
             const cplx p = cmul(delw, ie);
             cplx m = mu;
+            bool branch1 = false;
             if (nv > 0) {
                 const double delwr = cmul(delw, cconj(delw)).re;
                 cplx ssx = czero();
                 if ((wdiffr > 0.25) && (delwr < limitone)) {
+                    branch1 = true;
                     m = cadd(mu, mo);  // sch = P for the occupied bands
                     ssx = cscale2(om2, wx2[f], rwx4[f]);
                 } else if (delwr > kTo1) {
@@ -219,6 +228,10 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             }
             const cplx t = cmul_fast(p, m);
             sach[f] = cadd(sach[f], t);
+            if (a.p2 && iw0 + f == a.acht_iw) {
+                a.p2[row + ig] = p;
+                if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
+            }
         }
     }
     const double vc = a.vcoul[igp_of(a.idx, j, ncouls)];
@@ -232,6 +245,157 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             a.col_ach[o] = cscale(cscale(s, 0.5), vc);
             a.col_asx[o] = cscale(x, a.occ * vc);
         }
+    }
+}
+
+// acht_n1_loc (gppComplex.cpp:381, sch_array[2] of each band and column):
+//   acht[n1] = sum_j 0.5 vcoul[igp(j)] conj(aqsm[n1,igp(j)]) sum_ig aqsn[n1,ig] R[j,ig]
+// with R = p2 (P = delw*I_eps at that frequency) for the unoccupied bands and
+// R = p2 masked by br1 (sch of ssxt_scht_solver, :37) for the occupied ones.
+// Y = R * aqsn^T is a second Gram-type product (rows j, columns n1, K = G);
+// one block computes a TJ x TN tile of Y over G-vectors [k_begin, k_end) on
+// v_mfma_f64_16x16x4f64 and reduces it at once to the column sums
+// sum_j w_j conj(bg[n1][j]) Y[j][n1], written to part[n1] (one entry per
+// column of the tile).  Operand tiles are staged K-fastest with one complex
+// of padding per row, so both the coalesced global loads and the MFMA reads
+// hit distinct LDS banks.
+constexpr int kAK = 8;  // G-vectors per LDS stage of the acht product
+
+template <int WJN, int WIN> struct AchtCfg {
+    static constexpr int TJ = kWJ * WJN, TN = kWI * WIN, NT = 64 * WJN * WIN;
+};
+
+template <int WJN, int WIN, bool MASK>
+__device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const uint8_t *__restrict__ br1,
+                                              const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
+                                              const int32_t *__restrict__ idx, const double *__restrict__ vcoul,
+                                              int ngpown, int ncouls, int n1_end, int k_begin, int k_end, int j0,
+                                              int c0, cplx (*sR)[AchtCfg<WJN, WIN>::TJ][kAK + 1],
+                                              cplx (*sA)[AchtCfg<WJN, WIN>::TN][kAK + 1],
+                                              cplx (*red)[AchtCfg<WJN, WIN>::TN], cplx *__restrict__ part)
+{
+    typedef AchtCfg<WJN, WIN> C;
+    constexpr int TJ = C::TJ, TN = C::TN, NT = C::NT;
+    constexpr int PR = kAK * TJ / NT, PA = kAK * TN / NT;
+    static_assert(PR * NT == kAK * TJ && PA * NT == kAK * TN, "stage must split evenly");
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int wjn = wave / WIN;
+    const int wj = wjn * kWJ;
+    const int wi = (wave % WIN) * kWI;
+
+    v4d acc_re[4][2], acc_im[4][2];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc_re[s][t] = v4d{0, 0, 0, 0};
+            acc_im[s][t] = v4d{0, 0, 0, 0};
+        }
+
+    cplx stR[PR], stA[PA];
+    auto load_stage = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PR; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int r = e / kAK, k = k0 + e % kAK, j = j0 + r;
+            const size_t o = (size_t)j * ncouls + k;
+            cplx v = czero();
+            if (k < k_end && j < ngpown && (!MASK || br1[o])) v = p2[o];
+            stR[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int r = e / kAK, k = k0 + e % kAK, n1 = c0 + r;
+            stA[q] = (k < k_end && n1 < n1_end) ? aqsn[(size_t)n1 * ncouls + k] : czero();
+        }
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PR; ++q) {
+            const int e = threadIdx.x + q * NT;
+            sR[buf][e / kAK][e % kAK] = stR[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = threadIdx.x + q * NT;
+            sA[buf][e / kAK][e % kAK] = stA[q];
+        }
+    };
+
+    const int nst = (k_end - k_begin + kAK - 1) / kAK;
+    load_stage(k_begin);
+    store_stage(0);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) load_stage(k_begin + (st + 1) * kAK);
+#pragma unroll
+        for (int kk = 0; kk < kAK; kk += 4) {
+            const int kr = kk + lane / 16;
+            double rre[4], rim[4], rneg[4], are[2], aim[2];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cplx r = sR[buf][wj + 16 * s + lane % 16][kr];
+                rre[s] = r.re;
+                rim[s] = r.im;
+                rneg[s] = -r.im;
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const cplx a = sA[buf][wi + 16 * t + lane % 16][kr];
+                are[t] = a.re;
+                aim[t] = a.im;
+            }
+            // R * a: re = rr*ar - ri*ai, im = rr*ai + ri*ar
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rre[s], are[t], acc_re[s][t], 0, 0, 0);
+                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rneg[s], aim[t], acc_re[s][t], 0, 0, 0);
+                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rre[s], aim[t], acc_im[s][t], 0, 0, 0);
+                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rim[s], are[t], acc_im[s][t], 0, 0, 0);
+                }
+        }
+        if (st + 1 < nst) store_stage(buf ^ 1);
+        __syncthreads();
+    }
+
+    // Epilogue: result r of lane l is Y[j][n1] with j = 4r + l/16 + 16s + wj,
+    // n1 = l%16 + 16t + wi.  Weight by 0.5 vcoul[igp(j)] conj(bg[n1][j]), sum
+    // the wave's 64 rows (fixed order: s, r, then the xor butterfly over
+    // l/16), then the WJN waves of one column range in wave order.
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = wi + 16 * t + lane % 16;
+        const int n1 = c0 + col;
+        cplx sum = czero();
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = j0 + wj + 16 * s + 4 * r + lane / 16;
+                if (j < ngpown && n1 < n1_end) {
+                    const double w = 0.5 * vcoul[igp_of(idx, j, ncouls)];
+                    const cplx b = bg[(size_t)n1 * ngpown + j];
+                    const cplx y = cplx{acc_re[s][t][r], acc_im[s][t][r]};
+                    sum = cadd(sum, cscale(cmul_fast(cconj(b), y), w));
+                }
+            }
+        sum.re += __shfl_xor(sum.re, 16);
+        sum.im += __shfl_xor(sum.im, 16);
+        sum.re += __shfl_xor(sum.re, 32);
+        sum.im += __shfl_xor(sum.im, 32);
+        if (lane < 16) red[wjn][col] = sum;
+    }
+    __syncthreads();
+    for (int col = threadIdx.x; col < TN; col += NT) {
+        cplx s = czero();
+#pragma unroll
+        for (int w = 0; w < WJN; ++w) s = cadd(s, red[w][col]);
+        if (c0 + col < n1_end) part[c0 + col] = s;
     }
 }
 

@@ -99,6 +99,7 @@ struct GramKind {
     int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
     int red_chunk = 8192;  // G-vectors per pole-reduce block
     int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
+    bool acht = false;     // also produce acht_n1_loc (a second product, R * aqsn^T)
 };
 constexpr int kGramTiles = 4;
 

@@ -279,7 +279,8 @@ class Context:
 
     def sigma(self, form: str = "direct", acht_n1_loc: Optional[bool] = None) -> Dict[str, np.ndarray]:
         """Evaluates the band loop.  acht_n1_loc (per-band sums) is requested
-        by default from the direct forms only; the factored forms reject it."""
+        by default from the direct forms only: the factored forms compute it
+        with a second product of the Gram product's size, so it is opt-in."""
         if self.problem is None:
             raise GppError(GPP_ERR_NO_INPUTS, "no inputs")
         if acht_n1_loc is None:

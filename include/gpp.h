@@ -208,7 +208,10 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
 /*
  * Evaluates achtemp/asxtemp (and the optional outputs) for the inputs last
  * uploaded or bound, with the given form.  Synchronous.  The GPU time of the
- * call is available from gpp_ctx_last_timing.
+ * call is available from gpp_ctx_last_timing.  With the factored forms,
+ * acht_n1_loc costs a second product of the size of the Gram product (FP64
+ * MFMA whatever the form) and more workspace (16 + 1 bytes per
+ * ngpown x ncouls element); pass NULL when it is not needed.
  */
 gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out);
 

@@ -53,13 +53,37 @@ def test_factored_readme_golden(ctx, oracle, ncouls, form):
     assert_close(out, oracle.sigma(arrs, nvband=2, threads=8), TOL_FACTORED, KEYS)
 
 
-def test_factored_rejects_acht_n1_loc(ctx):
-    import pygpp
-    arrs = gpp_inputs.seeded_inputs(4, 64, 4, seed=2)
-    ctx.upload(problem_of(arrs, 1), arrs)
-    with pytest.raises(pygpp.GppError) as e:
-        ctx.sigma("factored-f64", acht_n1_loc=True)
-    assert e.value.status == pygpp.GPP_ERR_UNSUPPORTED
+ACHT_SHAPES = SHAPES + [
+    (300, 9, 1500, 300, 3),   # several column tiles of both band ranges, 40+ occupied-tile columns
+    (40, 40, 600, 30, 3),     # every band occupied: no unoccupied launch
+    (70, 0, 2100, 129, 4),    # no occupied band; j tiles with one live row
+]
+
+
+@pytest.mark.parametrize("shape", ACHT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_acht_n1_loc_vs_oracle(ctx, oracle, shape, form):
+    """acht_n1_loc from the second product (pole factor of frequency 2, branch
+    mask for the occupied bands) against the oracle's per-band sums; asking for
+    it leaves the other outputs bit-identical."""
+    nb, nv, ncouls, ngpown, nfreq = shape
+    arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=sum(shape) + 11,
+                                    dw=0.25 if nfreq > 5 else 1.0)
+    ctx.upload(problem_of(arrs, nv), arrs)
+    out = ctx.sigma(form, acht_n1_loc=True)
+    assert_close(out, oracle.sigma(arrs, nvband=nv, threads=8), TOL_FACTORED, KEYS + ("acht_n1_loc",))
+    plain = ctx.sigma(form)
+    for k in KEYS:
+        np.testing.assert_array_equal(out[k], plain[k])
+
+
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_acht_n1_loc_near_pole(ctx, oracle, form):
+    """Occupied-band branch decisions at frequency 2 straddle their thresholds."""
+    arrs = gpp_inputs.near_pole_inputs(40, 1500, 70, seed=5)
+    ctx.upload(problem_of(arrs, 6), arrs)
+    ref = oracle.sigma(arrs, nvband=6, threads=8)
+    assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
 
 
 def test_factored_full_c1_band_subset(ctx, oracle):
@@ -79,6 +103,8 @@ def test_factored_full_c1_vs_direct_strict(ctx):
     for form in FACTORED:
         out = ctx.sigma(form)
         assert_close(out, ref, TOL_FACTORED, KEYS)
+        if form == "factored-f64":
+            assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
         again = ctx.sigma(form)
         for k in KEYS:
             np.testing.assert_array_equal(out[k], again[k])
@@ -147,8 +173,7 @@ def test_c0_cpu_config(ctx, oracle):
         ctx.upload(problem_of(arrs, 2), arrs)
         for form in ALL_FORMS:
             tol = 1e-12 if form == "direct" else TOL_FACTORED
-            keys = KEYS + (("acht_n1_loc",) if form == "direct" else ())
-            assert_close(ctx.sigma(form), ref, tol, keys)
+            assert_close(ctx.sigma(form, acht_n1_loc=True), ref, tol, KEYS + ("acht_n1_loc",))
 
 
 def test_c3_shape_subset_vs_oracle(ctx, oracle):
@@ -166,9 +191,9 @@ def test_c3_shape_subset_vs_oracle(ctx, oracle):
     ctx.upload(p, arrs)
     for form in ALL_FORMS:
         tol = 1e-12 if form == "direct" else TOL_FACTORED
-        out = ctx.sigma(form)
+        out = ctx.sigma(form, acht_n1_loc=True)
         assert_close_conditioned(out, seq, dd, tol)
-        assert_close(out, seq, tol, ("achstemp",) + (("acht_n1_loc",) if form == "direct" else ()))
+        assert_close(out, seq, tol, ("achstemp", "acht_n1_loc"))
 
 
 def test_c3_full_size_forms_agree(ctx):
