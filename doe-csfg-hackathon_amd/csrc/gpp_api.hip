@@ -166,6 +166,13 @@ struct gpp_ctx {
     int red_chunk = 8192;
     int red_fc = 8;
     int num_cus = 256;
+    int max_buckets = 32;
+    // band-dependent frequencies (gpp_set_band_frequencies)
+    bool per_band = false;
+    DevBuf wxb;                     // [nb][nfreq]
+    gpp::BucketPlan bplan;          // bands grouped by identical rows
+    gpp_form last_form = GPP_FORM_DIRECT;
+    bool have_last_form = false;
     // autotune results: (form, nb, nv, ncouls, ngpown, nfreq) -> (tile, chunk, fc)
     std::map<std::tuple<int, int, int, int, int, int>, std::tuple<int, int, int>> tuned;
 };
@@ -279,7 +286,7 @@ gpp_status gpp_ctx_destroy(gpp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->idx.release(); ctx->wx.release(); ctx->vcoul.release();
     ctx->aqsm.release(); ctx->aqsn.release(); ctx->ieps.release(); ctx->wtilde.release();
-    ctx->results.release(); ctx->flag.release();
+    ctx->results.release(); ctx->flag.release(); ctx->wxb.release();
     ctx->work.release();
     ctx->timer.destroy();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -315,6 +322,7 @@ gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_
     HIP_OR_FAIL(hipStreamSynchronize(s), "upload synchronize");
     ctx->prob = *prob;
     ctx->have_inputs = true;
+    ctx->per_band = false;
     return GPP_OK;
 }
 
@@ -345,6 +353,7 @@ gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t 
     dp.wtilde = reinterpret_cast<const cplx *>(d_wtilde_array);
     ctx->prob = *prob;
     ctx->have_inputs = true;
+    ctx->per_band = false;
     return GPP_OK;
 }
 
@@ -455,6 +464,45 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
     HIP_OR_FAIL(hipStreamSynchronize(s), "generate synchronize");
     ctx->prob = *prob;
     ctx->have_inputs = true;
+    ctx->per_band = false;
+    return GPP_OK;
+}
+
+gpp_status gpp_set_band_frequencies(gpp_ctx *ctx, const double *wx_bands)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    if (!wx_bands) {
+        ctx->per_band = false;
+        return GPP_OK;
+    }
+    const int nb = ctx->dp.nb, nf = ctx->dp.nfreq;
+    for (size_t i = 0; i < (size_t)nb * nf; ++i)
+        if (!(wx_bands[i] == wx_bands[i]))
+            return fail(GPP_ERR_INVALID_ARGUMENT, "wx_bands[%zu] is NaN", i);
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    HIP_OR_FAIL(ctx->wxb.ensure(sizeof(double) * nb * nf), "hipMalloc wx_bands");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->wxb.ptr, wx_bands, sizeof(double) * nb * nf, hipMemcpyHostToDevice, ctx->stream),
+                "H2D wx_bands");
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "wx_bands synchronize");
+    ctx->bplan = gpp::plan_buckets(wx_bands, nb, ctx->dp.nv, nf);
+    ctx->per_band = true;
+    return GPP_OK;
+}
+
+gpp_status gpp_ctx_band_buckets(const gpp_ctx *ctx, int *nbuckets)
+{
+    if (!ctx || !nbuckets) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *nbuckets = ctx->per_band ? (int)ctx->bplan.row.size() : 1;
+    return GPP_OK;
+}
+
+gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form)
+{
+    if (!ctx || !form) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!ctx->have_last_form) return fail(GPP_ERR_NO_INPUTS, "no gpp_sigma has run on this context");
+    *form = ctx->last_form;
     return GPP_OK;
 }
 
@@ -464,19 +512,38 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     if (!out || !out->achtemp || !out->asxtemp)
         return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
     if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    if (form < GPP_FORM_DIRECT || form > GPP_FORM_FACTORED_OZAKI_I8)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
     gpp_status st = set_device(ctx);
     if (st) return st;
-    const gpp::DevProblem &dp = ctx->dp;
+    // Band-dependent frequencies: the direct forms read a row per band; the
+    // factored forms factor each group of bands sharing a row (one group:
+    // the plain factored form on that row) and fall back to the direct form
+    // beyond max_buckets groups.
+    gpp::DevProblem dp = ctx->dp;
+    bool bucketed = false;
+    if (ctx->per_band) {
+        const int nbk = (int)ctx->bplan.row.size();
+        const bool factored = form != GPP_FORM_DIRECT && form != GPP_FORM_DIRECT_STRICT;
+        if (factored && nbk == 1) {
+            dp.wx = static_cast<const double *>(ctx->wxb.ptr);
+        } else {
+            dp.wx = static_cast<const double *>(ctx->wxb.ptr);
+            dp.wx_stride = dp.nfreq;
+            if (factored && nbk <= ctx->max_buckets) bucketed = true;
+            else if (factored) form = GPP_FORM_DIRECT;
+        }
+    }
     size_t need = 0;
     gpp::GramKind kind;
     switch (form) {
     case GPP_FORM_DIRECT:
     case GPP_FORM_DIRECT_STRICT: need = gpp::direct_workspace_bytes(dp); break;
-    case GPP_FORM_FACTORED_F64:
-    case GPP_FORM_FACTORED_OZAKI:
-    case GPP_FORM_FACTORED_OZAKI_I8: kind = gram_kind(ctx, form); kind.acht = out->acht_n1_loc != nullptr;
-        need = gpp::factored_workspace_bytes(dp, kind); break;
-    default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
+    default:
+        kind = gram_kind(ctx, form);
+        kind.acht = out->acht_n1_loc != nullptr;
+        need = bucketed ? gpp::bucketed_workspace_bytes(dp, kind, ctx->bplan) : gpp::factored_workspace_bytes(dp, kind);
+        break;
     }
     if (need == 0) return fail(GPP_ERR_UNSUPPORTED, "form %d is not available for this problem", (int)form);
     HIP_OR_FAIL(ctx->work.reserve(need), "workspace allocation");
@@ -488,9 +555,13 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     hipError_t e;
     if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT)
         e = gpp::run_direct(dp, form == GPP_FORM_DIRECT_STRICT, ctx->work.ptr, res, ctx->stream, &ctx->timer);
+    else if (bucketed)
+        e = gpp::run_factored_bucketed(dp, kind, ctx->bplan, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     else
         e = gpp::run_factored(dp, kind, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     if (e != hipSuccess) return fail_hip(e, "sigma launch");
+    ctx->last_form = form;
+    ctx->have_last_form = true;
     std::vector<cplx> host(res.count());
     HIP_OR_FAIL(hipMemcpyAsync(host.data(), res.base, sizeof(cplx) * res.count(), hipMemcpyDeviceToHost, ctx->stream),
                 "D2H results");
@@ -529,6 +600,11 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         if (value != 4 && value != 8) return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs must be 4 or 8, got %d", value);
         ctx->red_fc = value;
         return GPP_OK;
+    case GPP_OPT_MAX_BUCKETS:
+        if (value < 1 || value > (1 << 20))
+            return fail(GPP_ERR_INVALID_ARGUMENT, "max buckets must be in [1, 2^20], got %d", value);
+        ctx->max_buckets = value;
+        return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
 }
@@ -542,6 +618,7 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
     case GPP_OPT_GRAM_TILE: *value = ctx->gram_tile; return GPP_OK;
     case GPP_OPT_REDUCE_CHUNK: *value = ctx->red_chunk; return GPP_OK;
     case GPP_OPT_REDUCE_FREQS: *value = ctx->red_fc; return GPP_OK;
+    case GPP_OPT_MAX_BUCKETS: *value = ctx->max_buckets; return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
 }

@@ -38,6 +38,7 @@ struct DirectArgs {
     const int32_t *idx;
     const double *vcoul, *wx;
     int nb, nv, ncouls, ngpown, nfreq, njt;
+    int wx_stride;  // 0: shared wx row; nfreq: band n1 reads wx[n1 * nfreq + iw]
     double occ;
     cplx *part_ach;   // [nb][njt][nfreq]
     cplx *part_asx;   // [max(nv,1)][njt][nfreq]
@@ -73,7 +74,7 @@ DirectArgs make_args(const DevProblem &p, void *work)
     char *w = static_cast<char *>(work);
     DirectArgs a;
     a.aqsm = p.aqsm; a.aqsn = p.aqsn; a.ieps = p.ieps; a.wtilde = p.wtilde;
-    a.idx = p.idx; a.vcoul = p.vcoul; a.wx = p.wx;
+    a.idx = p.idx; a.vcoul = p.vcoul; a.wx = p.wx; a.wx_stride = p.wx_stride;
     a.nb = p.nb; a.nv = p.nv; a.ncouls = p.ncouls; a.ngpown = p.ngpown; a.nfreq = p.nfreq;
     a.njt = (p.ngpown + kBJ - 1) / kBJ;
     a.occ = p.occ;
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(kNW *kWave) void k_unocc(DirectArgs a)
     }
     double wxv[FC];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) wxv[f] = (iw0 + f < a.nfreq) ? a.wx[iw0 + f] : 0.0;
+    for (int f = 0; f < FC; ++f)
+        wxv[f] = (band_ok && iw0 + f < a.nfreq) ? a.wx[(size_t)n1 * a.wx_stride + iw0 + f] : 0.0;
 
     cplx acc[kBJ][FC];
 #pragma unroll
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(kOccWaves *kWave) void k_occ(DirectArgs a)
     double wxv[FC], wx2[FC], rwx4[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
-        wxv[f] = (iw0 + f < a.nfreq) ? a.wx[iw0 + f] : 1.0;
+        wxv[f] = (iw0 + f < a.nfreq) ? a.wx[(size_t)n1 * a.wx_stride + iw0 + f] : 1.0;
         // std::pow(wxt,2) and std::pow(cden,2), gppComplex.cpp:38-40
         wx2[f] = wxv[f] * wxv[f];
         rwx4[f] = 1.00 / (wx2[f] * wx2[f]);

@@ -15,6 +15,10 @@
 //   5. k_acht_f64      second FP64 MFMA product Y = R * aqsn^T, split over
 //                      G-vectors, reduced per band in its epilogue
 //   6. k_acht_final    fixed-order sum of the per-tile band partials.
+#include <string.h>
+
+#include <algorithm>
+
 #include "gpp_factored_dev.hpp"
 
 namespace gpp {
@@ -206,6 +210,7 @@ hipError_t launch_pole_reduce(const ReduceArgs &ra, int red_fc, hipStream_t s)
 hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const DevResults &res, hipStream_t s,
                         StageTimer *timer)
 {
+    if (p.wx_stride != 0) return hipErrorInvalidValue;  // the band sum factors only for a shared wx row
     const FactLayout L = fact_layout(p, kind);
     char *w = static_cast<char *>(work);
     cplx *bg = reinterpret_cast<cplx *>(w + L.bg);
@@ -266,6 +271,172 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
         GPP_TRY(hipGetLastError());
         if (timer) timer->mark(s, "factored_acht");
     }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Band-dependent frequencies.  The band sum factors over any set of bands
+// that share one wx row, so bands are grouped by identical rows ("buckets")
+// and each bucket is one factored sub-problem: its bands are gathered into a
+// contiguous block (occupied bands first), the sub-problem runs through
+// run_factored with that bucket's row, and the per-bucket results are summed
+// in bucket order (acht_n1_loc scattered back to the original band order).
+
+namespace {
+
+__global__ void k_permute_rows(const cplx *__restrict__ src, const int32_t *__restrict__ perm, int nrows,
+                               int ncols, cplx *__restrict__ dst)
+{
+    const size_t n = (size_t)nrows * ncols;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t / ncols, c = t % ncols;
+        dst[t] = src[(size_t)perm[r] * ncols + c];
+    }
+}
+
+// One block.  tab[3b..3b+2] = (result offset, first permuted row, bands) of
+// bucket b; bres holds each bucket's DevResults back to back.
+__global__ void k_bucket_combine(const cplx *__restrict__ bres, const int32_t *__restrict__ tab,
+                                 const int32_t *__restrict__ perm, int nbuckets, int nfreq, int nb,
+                                 cplx *__restrict__ out)
+{
+    for (int iw = threadIdx.x; iw < 2 * nfreq; iw += blockDim.x) {
+        cplx s = czero();
+        for (int b = 0; b < nbuckets; ++b) s = cadd(s, bres[tab[3 * b] + iw]);
+        out[iw] = s;  // achtemp[iw] for iw < nfreq, then asxtemp
+    }
+    if (threadIdx.x == 0) {
+        cplx s = czero();
+        for (int b = 0; b < nbuckets; ++b) s = cadd(s, bres[tab[3 * b] + 2 * nfreq + tab[3 * b + 2]]);
+        out[2 * nfreq + nb] = s;  // achstemp
+    }
+    for (int b = 0; b < nbuckets; ++b)
+        for (int k = threadIdx.x; k < tab[3 * b + 2]; k += blockDim.x)
+            out[2 * nfreq + perm[tab[3 * b + 1] + k]] = bres[tab[3 * b] + 2 * nfreq + k];
+}
+
+struct BucketLayout {
+    size_t aqsm, aqsn, perm, tab, bres, sub, total;
+};
+
+DevProblem bucket_problem(const DevProblem &p, const BucketPlan &bp, int b, const cplx *aqsm_p, const cplx *aqsn_p)
+{
+    DevProblem q = p;
+    q.nb = bp.size[b];
+    q.nv = bp.nocc[b];
+    q.aqsm = aqsm_p + (size_t)bp.start[b] * p.ncouls;
+    q.aqsn = aqsn_p + (size_t)bp.start[b] * p.ncouls;
+    q.wx = p.wx + (size_t)bp.row[b] * p.nfreq;
+    q.wx_stride = 0;
+    return q;
+}
+
+BucketLayout bucket_layout(const DevProblem &p, const GramKind &k, const BucketPlan &bp)
+{
+    BucketLayout L;
+    const int nbk = (int)bp.size.size();
+    size_t off = 0;
+    L.aqsm = off; off += align256(sizeof(cplx) * (size_t)p.nb * p.ncouls);
+    L.aqsn = off; off += align256(sizeof(cplx) * (size_t)p.nb * p.ncouls);
+    L.perm = off; off += align256(sizeof(int32_t) * (size_t)p.nb);
+    L.tab = off;  off += align256(sizeof(int32_t) * 3 * (size_t)nbk);
+    size_t sub = 0;
+    for (int b = 0; b < nbk; ++b) {
+        const DevProblem q = bucket_problem(p, bp, b, nullptr, nullptr);
+        sub = std::max(sub, factored_workspace_bytes(q, k));
+    }
+    L.bres = off; off += align256(sizeof(cplx) * bp.nres);
+    L.sub = off;  off += sub;
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+BucketPlan plan_buckets(const double *wx_rows, int nb, int nv, int nfreq)
+{
+    BucketPlan bp;
+    std::vector<int> bucket(nb, -1);
+    for (int n1 = 0; n1 < nb; ++n1) {
+        const double *r = wx_rows + (size_t)n1 * nfreq;
+        for (int b = 0; b < (int)bp.row.size(); ++b)
+            if (memcmp(r, wx_rows + (size_t)bp.row[b] * nfreq, sizeof(double) * nfreq) == 0) {
+                bucket[n1] = b;
+                break;
+            }
+        if (bucket[n1] < 0) {
+            bucket[n1] = (int)bp.row.size();
+            bp.row.push_back(n1);
+        }
+    }
+    const int nbk = (int)bp.row.size();
+    bp.start.assign(nbk, 0);
+    bp.size.assign(nbk, 0);
+    bp.nocc.assign(nbk, 0);
+    bp.tab.assign(3 * (size_t)nbk, 0);
+    bp.nres = 0;
+    for (int b = 0; b < nbk; ++b) {
+        bp.start[b] = (int)bp.perm.size();
+        for (int pass = 0; pass < 2; ++pass)  // occupied bands first, each in band order
+            for (int n1 = 0; n1 < nb; ++n1)
+                if (bucket[n1] == b && (n1 < nv) == (pass == 0)) {
+                    bp.perm.push_back(n1);
+                    ++bp.size[b];
+                    if (pass == 0) ++bp.nocc[b];
+                }
+        bp.tab[3 * b] = (int32_t)bp.nres;
+        bp.tab[3 * b + 1] = bp.start[b];
+        bp.tab[3 * b + 2] = bp.size[b];
+        bp.nres += (size_t)2 * nfreq + bp.size[b] + 1;
+    }
+    return bp;
+}
+
+size_t bucketed_workspace_bytes(const DevProblem &p, GramKind kind, const BucketPlan &bp)
+{
+    return bucket_layout(p, kind, bp).total;
+}
+
+hipError_t run_factored_bucketed(const DevProblem &p, GramKind kind, const BucketPlan &bp, void *work,
+                                 const DevResults &res, hipStream_t s, StageTimer *timer)
+{
+    if (p.wx_stride != p.nfreq || (int)bp.perm.size() != p.nb || bp.tab.size() != 3 * bp.size.size())
+        return hipErrorInvalidValue;
+    const BucketLayout L = bucket_layout(p, kind, bp);
+    char *w = static_cast<char *>(work);
+    cplx *aqsm_p = reinterpret_cast<cplx *>(w + L.aqsm);
+    cplx *aqsn_p = reinterpret_cast<cplx *>(w + L.aqsn);
+    int32_t *perm = reinterpret_cast<int32_t *>(w + L.perm);
+    int32_t *tab = reinterpret_cast<int32_t *>(w + L.tab);
+    cplx *bres = reinterpret_cast<cplx *>(w + L.bres);
+    const int nbk = (int)bp.size.size();
+
+    if (timer) timer->begin(s);
+    GPP_TRY(hipMemcpyAsync(perm, bp.perm.data(), sizeof(int32_t) * p.nb, hipMemcpyHostToDevice, s));
+    GPP_TRY(hipMemcpyAsync(tab, bp.tab.data(), sizeof(int32_t) * 3 * nbk, hipMemcpyHostToDevice, s));
+    {
+        const size_t n = (size_t)p.nb * p.ncouls;
+        size_t blocks = (n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)blocks), dim3(256), 0, s, p.aqsm, perm, p.nb, p.ncouls,
+                           aqsm_p);
+        hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)blocks), dim3(256), 0, s, p.aqsn, perm, p.nb, p.ncouls,
+                           aqsn_p);
+        GPP_TRY(hipGetLastError());
+    }
+    if (timer) timer->mark(s, "bucket_permute");
+    for (int b = 0; b < nbk; ++b) {
+        const DevProblem q = bucket_problem(p, bp, b, aqsm_p, aqsn_p);
+        DevResults r;
+        r.base = bres + bp.tab[3 * b];
+        r.nfreq = p.nfreq;
+        r.nb = q.nb;
+        GPP_TRY(run_factored(q, kind, w + L.sub, r, s, nullptr));
+    }
+    if (timer) timer->mark(s, "bucket_factored");
+    hipLaunchKernelGGL(k_bucket_combine, dim3(1), dim3(256), 0, s, bres, tab, perm, nbk, p.nfreq, p.nb, res.base);
+    GPP_TRY(hipGetLastError());
+    if (timer) timer->mark(s, "bucket_combine");
     return hipSuccess;
 }
 

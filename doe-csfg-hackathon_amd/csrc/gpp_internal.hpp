@@ -17,7 +17,8 @@ struct DevProblem {
     int nb = 0, nv = 0, ncouls = 0, ngpown = 0, nfreq = 0;
     double occ = 1.0;
     const int32_t *idx = nullptr;  // [ngpown]
-    const double *wx = nullptr;    // [nfreq]
+    const double *wx = nullptr;    // [nfreq], or [nb][nfreq] when wx_stride == nfreq
+    int wx_stride = 0;             // 0: one wx row for every band; nfreq: a row per band
     const double *vcoul = nullptr; // [ncouls]
     const cplx *aqsm = nullptr;    // [nb][ncouls]
     const cplx *aqsn = nullptr;    // [nb][ncouls]
@@ -106,6 +107,22 @@ constexpr int kGramTiles = 4;
 size_t factored_workspace_bytes(const DevProblem &p, GramKind kind);
 hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const DevResults &res,
                         hipStream_t s, StageTimer *timer);
+
+// Band-dependent frequencies (gpp_factored.hip): bands grouped by identical
+// wx rows; bucket b holds bands perm[start[b] .. start[b]+size[b]), its
+// nocc[b] occupied bands first, and uses row `row[b]` (its first band).
+struct BucketPlan {
+    std::vector<int32_t> perm;
+    std::vector<int> start, size, nocc, row;
+    std::vector<int32_t> tab;  // per bucket: result offset, start, size
+    size_t nres = 0;           // complex results of all buckets (DevResults each)
+};
+BucketPlan plan_buckets(const double *wx_rows, int nb, int nv, int nfreq);
+size_t bucketed_workspace_bytes(const DevProblem &p, GramKind kind, const BucketPlan &bp);
+// p.wx is [nb][nfreq] (wx_stride == nfreq); the host vectors of bp must stay
+// alive until the stream has run the call.
+hipError_t run_factored_bucketed(const DevProblem &p, GramKind kind, const BucketPlan &bp, void *work,
+                                 const DevResults &res, hipStream_t s, StageTimer *timer);
 
 // Ozaki Gram product (gpp_ozaki.hip): Mu = conj(Bg)^T aqsn over bands
 // [nv, nb), written to mout [ngpown][ncouls].

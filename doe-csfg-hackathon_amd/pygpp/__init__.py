@@ -38,6 +38,7 @@ GPP_OPT_OZAKI_LAYOUT = 2
 GPP_OPT_GRAM_TILE = 3
 GPP_OPT_REDUCE_CHUNK = 4
 GPP_OPT_REDUCE_FREQS = 5
+GPP_OPT_MAX_BUCKETS = 6
 
 # gpp_form
 FORMS = {
@@ -47,6 +48,7 @@ FORMS = {
     "factored-ozaki": 3,
     "factored-ozaki-i8": 4,
 }
+FORM_NAMES = {v: k for k, v in FORMS.items()}
 
 # Every symbol include/gpp.h declares.
 EXPORTED_SYMBOLS = (
@@ -60,6 +62,9 @@ EXPORTED_SYMBOLS = (
     "gpp_fill_seeded",
     "gpp_generate_seeded",
     "gpp_sigma",
+    "gpp_set_band_frequencies",
+    "gpp_ctx_band_buckets",
+    "gpp_ctx_last_form",
     "gpp_ctx_set_option",
     "gpp_ctx_get_option",
     "gpp_ctx_last_timing",
@@ -135,6 +140,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_fill_seeded.argtypes = [P(GppProblem), ctypes.c_uint64, dbl, dbl] + [vp] * 7
     lib.gpp_generate_seeded.argtypes = [vp, P(GppProblem), ctypes.c_uint64, dbl, dbl]
     lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
+    lib.gpp_set_band_frequencies.argtypes = [vp, vp]
+    lib.gpp_ctx_band_buckets.argtypes = [vp, P(i32)]
+    lib.gpp_ctx_last_form.argtypes = [vp, P(i32)]
     lib.gpp_ctx_last_timing.argtypes = [vp, P(dbl), P(dbl), i32, P(i32)]
     lib.gpp_ctx_set_option.argtypes = [vp, i32, i32]
     lib.gpp_ctx_get_option.argtypes = [vp, i32, P(i32)]
@@ -255,9 +263,13 @@ class Context:
             pass
 
     def upload(self, p: Problem, arrs: Dict[str, np.ndarray]) -> None:
+        """Copies the inputs to the device.  A 2-D wx_array [number_bands][nfreq]
+        uploads its first row as the shared wx_array and then sets the rows as
+        band-dependent frequencies (set_band_frequencies)."""
+        wx = np.asarray(arrs["wx_array"], np.float64)
         a = {
             "inv_igp_index": _contig(arrs["inv_igp_index"], np.int32),
-            "wx_array": _contig(arrs["wx_array"], np.float64),
+            "wx_array": _contig(wx[0] if wx.ndim == 2 else wx, np.float64),
             "vcoul": _contig(arrs["vcoul"], np.float64),
         }
         for k in ("aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array"):
@@ -265,6 +277,30 @@ class Context:
         prob = p.c()
         _check(load().gpp_upload(self._h, ctypes.byref(prob), *[_ptr(a[k]) for k in _INPUT_ORDER]))
         self.problem = p
+        if wx.ndim == 2:
+            self.set_band_frequencies(wx)
+
+    def set_band_frequencies(self, wx_bands: Optional[np.ndarray]) -> None:
+        """Band n1 uses wx_bands[n1] ([number_bands][nfreq]); None restores
+        the shared wx_array."""
+        if wx_bands is None:
+            _check(load().gpp_set_band_frequencies(self._h, None))
+            return
+        w = _contig(wx_bands, np.float64)
+        p = self.problem
+        if p is None or w.shape != (p.number_bands, p.nfreq):
+            raise ValueError("wx_bands must be [number_bands][nfreq] of the uploaded problem")
+        _check(load().gpp_set_band_frequencies(self._h, _ptr(w)))
+
+    def band_buckets(self) -> int:
+        n = ctypes.c_int()
+        _check(load().gpp_ctx_band_buckets(self._h, ctypes.byref(n)))
+        return n.value
+
+    def last_form(self) -> str:
+        f = ctypes.c_int()
+        _check(load().gpp_ctx_last_form(self._h, ctypes.byref(f)))
+        return FORM_NAMES[f.value]
 
     def bind_device(self, p: Problem, ptrs: Dict[str, int]) -> None:
         """Binds caller-owned device buffers (integer device addresses)."""

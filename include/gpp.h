@@ -112,7 +112,12 @@ typedef enum {
      * (default 8192). */
     GPP_OPT_REDUCE_CHUNK = 4,
     /* Frequencies per pole-reduce pass when nfreq > 4: 4 or 8 (default 8). */
-    GPP_OPT_REDUCE_FREQS = 5
+    GPP_OPT_REDUCE_FREQS = 5,
+    /* With band-dependent frequencies (gpp_set_band_frequencies): the most
+     * groups of bands sharing one frequency row that the factored forms
+     * factor group by group; beyond it they run the direct form (default
+     * 32). */
+    GPP_OPT_MAX_BUCKETS = 6
 } gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).
@@ -214,6 +219,28 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
  * ngpown x ncouls element); pass NULL when it is not needed.
  */
 gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out);
+
+/*
+ * Band-dependent frequencies: band n1 uses wx_bands[n1 * nfreq + iw]
+ * (host memory, [number_bands][nfreq], copied) instead of the shared
+ * wx_array, as BerkeleyGW's wx = e_lk - e_n1kq(n1) + dw*(iw-1); the mini-app
+ * fixes one e_n1kq for all bands (gppComplex.cpp:326), so its wx_array is
+ * band-independent.  Applies to the inputs last uploaded or bound, until the
+ * next gpp_upload / gpp_bind_device / gpp_generate_seeded; NULL restores the
+ * shared wx_array.  The direct forms read the band's row.  The band sum
+ * factors only over bands sharing a row, so the factored forms group the
+ * bands by identical rows and factor each group (one group: exactly the
+ * plain factored form on that row); with more than GPP_OPT_MAX_BUCKETS
+ * groups they run GPP_FORM_DIRECT instead (gpp_ctx_last_form tells which).
+ * The batched entry points take one wx_array per instance.
+ */
+gpp_status gpp_set_band_frequencies(gpp_ctx *ctx, const double *wx_bands);
+
+/* Number of distinct frequency rows of the current inputs (1 when shared). */
+gpp_status gpp_ctx_band_buckets(const gpp_ctx *ctx, int *nbuckets);
+
+/* The form the last gpp_sigma on this context actually ran. */
+gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form);
 
 /* Sets / reads a context option (gpp_option). */
 gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value);

@@ -30,6 +30,12 @@
  *                                reference harness on seeded fixtures.
  *   oracle_sigma_dd           -- agrees with oracle_sigma to rounding and
  *                                with the exact-rational brute force.
+ *   oracle_sigma_wxb          -- equal rows reproduce oracle_sigma bitwise;
+ *                                distinct rows: each band's acht_n1_loc equals,
+ *                                bitwise, the one-band run with that band's
+ *                                row (oracle and compiled reference harness),
+ *                                and achtemp/asxtemp/achstemp equal the sum of
+ *                                the one-band runs to rounding.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -39,6 +45,13 @@
 #endif
 
 typedef struct { double re, im; } oc_cplx;
+
+int oracle_sigma_wxb(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
+                     const int *inv_igp_index, const double *wx_array, int wx_stride, const double *vcoul,
+                     const double *aqsmtemp, const double *aqsntemp,
+                     const double *I_eps_array, const double *wtilde_array,
+                     double *achtemp_out, double *asxtemp_out, double *acht_n1_loc_out,
+                     double *achstemp_out, int nthreads);
 
 /* GPPComplex(x, y), Complex.h:27-30 */
 static inline oc_cplx oc_make(double re, double im) { oc_cplx r; r.re = re; r.im = im; return r; }
@@ -244,8 +257,28 @@ int oracle_sigma(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
                  double *achtemp_out, double *asxtemp_out, double *acht_n1_loc_out,
                  double *achstemp_out, int nthreads)
 {
+    return oracle_sigma_wxb(nb, nv, ncouls, ngpown, nfreq, occ, inv_igp_index, wx_array, 0, vcoul, aqsmtemp,
+                            aqsntemp, I_eps_array, wtilde_array, achtemp_out, asxtemp_out, acht_n1_loc_out,
+                            achstemp_out, nthreads);
+}
+
+/*
+ * The same loop with band-dependent frequencies: band n1 uses
+ * wx[n1 * wx_stride + iw] (wx_stride 0: the reference's shared wx_array;
+ * wx_stride nfreq: one row per band, as BerkeleyGW's wx = e_lk - e_n1kq(n1)
+ * + dw*(iw-1), which the mini-app fixes to one e_n1kq, gppComplex.cpp:326).
+ * Only the wxt of :354 changes; everything else is the loop above.
+ */
+int oracle_sigma_wxb(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
+                     const int *inv_igp_index, const double *wx_array, int wx_stride, const double *vcoul,
+                     const double *aqsmtemp, const double *aqsntemp,
+                     const double *I_eps_array, const double *wtilde_array,
+                     double *achtemp_out, double *asxtemp_out, double *acht_n1_loc_out,
+                     double *achstemp_out, int nthreads)
+{
     int rc = oc_check(nb, nv, ncouls, ngpown, nfreq, inv_igp_index);
     if (rc) return rc;
+    if (wx_stride != 0 && wx_stride != nfreq) return 1;
     const int acht_iw = oc_acht_iw(nfreq);
     oc_cplx *achtemp = calloc((size_t)nfreq, sizeof(oc_cplx));
     oc_cplx *asxtemp = calloc((size_t)nfreq, sizeof(oc_cplx));
@@ -277,7 +310,7 @@ int oracle_sigma(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
                 sch_array[i] = oc_make(0.0, 0.0);
             }
             for (int iw = 0; iw < nfreq; ++iw) {
-                double wxt = wx_array[iw];
+                double wxt = wx_array[(size_t)n1 * wx_stride + iw];
                 if (flag_occ) {                                              /* :355-364 */
                     oc_cplx scht = oc_make(0.0, 0.0), ssxt = oc_make(0.0, 0.0);
                     oc_flag_occ(wxt, wtilde_array, my_igp, n1, aqsmtemp, aqsntemp, I_eps_array,

@@ -39,6 +39,9 @@ def load_oracle() -> ctypes.CDLL:
         vp = ctypes.c_void_p
         lib.oracle_sigma.argtypes = _sigma_argtypes([vp, vp, vp, vp, ctypes.c_int])
         lib.oracle_sigma.restype = ctypes.c_int
+        lib.oracle_sigma_wxb.argtypes = ([ctypes.c_int] * 5 + [ctypes.c_double, vp, vp, ctypes.c_int]
+                                         + [vp] * 5 + [vp, vp, vp, vp, ctypes.c_int])
+        lib.oracle_sigma_wxb.restype = ctypes.c_int
         lib.oracle_sigma_dd.argtypes = _sigma_argtypes([vp, vp, ctypes.c_int])
         lib.oracle_sigma_dd.restype = ctypes.c_int
         lib.oracle_pole_terms.argtypes = [ctypes.c_double, vp, vp, vp, vp]
@@ -75,21 +78,30 @@ def _prep(arrs):
 def _sizes(arrs, nvband, occ):
     nb, ncouls = arrs["aqsmtemp"].shape
     ngpown = arrs["I_eps_array"].shape[0]
-    nfreq = arrs["wx_array"].shape[0]
+    nfreq = arrs["wx_array"].shape[-1]
     return [nb, nvband, ncouls, ngpown, nfreq, ctypes.c_double(occ)]
 
 
 def sigma(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: int = 1,
           use_ref: bool = False) -> Dict[str, np.ndarray]:
-    """Runs the oracle (or the compiled reference harness) on the inputs."""
+    """Runs the oracle (or the compiled reference harness) on the inputs.
+    A 2-D wx_array [number_bands][nfreq] gives each band its own frequencies
+    (oracle only: the reference shares one wx_array between all bands)."""
     ins = _prep(arrs)
     sizes = _sizes(arrs, nvband, occ)
+    per_band = np.ndim(arrs["wx_array"]) == 2
     nb, nfreq = sizes[0], sizes[4]
     out = {"achtemp": np.zeros(nfreq, np.complex128), "asxtemp": np.zeros(nfreq, np.complex128),
            "acht_n1_loc": np.zeros(nb, np.complex128), "achstemp": np.zeros(1, np.complex128)}
     optr = [out[k].ctypes.data for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")]
     iptr = [a.ctypes.data for a in ins]
-    if use_ref:
+    if per_band:
+        if use_ref:
+            raise ValueError("the reference harness takes one wx_array for all bands")
+        if np.shape(arrs["wx_array"])[0] != nb:
+            raise ValueError("per-band wx_array must be [number_bands][nfreq]")
+        rc = load_oracle().oracle_sigma_wxb(*sizes, iptr[0], iptr[1], nfreq, *iptr[2:], *optr, threads)
+    elif use_ref:
         rc = load_ref().ref_sigma(*sizes, *iptr, *optr)
     else:
         rc = load_oracle().oracle_sigma(*sizes, *iptr, *optr, threads)
@@ -101,6 +113,8 @@ def sigma(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: i
 
 def sigma_dd(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: int = 1):
     """Compensated (double-double accumulated) achtemp, asxtemp."""
+    if np.ndim(arrs["wx_array"]) != 1:
+        raise ValueError("sigma_dd takes one wx_array for all bands")
     ins = _prep(arrs)
     sizes = _sizes(arrs, nvband, occ)
     nfreq = sizes[4]

@@ -72,3 +72,18 @@ def near_pole_inputs(nb, ncouls, ngpown, seed=0):
     a["wtilde_array"] = wt
     a["wx_array"] = np.array([399.4, 4.0, 0.1])
     return a
+
+
+def per_band_wx(nb, nfreq, seed=0, levels=None, e_lk=10.0, dw=1.0):
+    """Band-dependent frequencies wx[n1][iw] = e_lk - e_n1kq[n1] + dw*(iw-1),
+    clamped at 1e-6 as gppComplex.cpp:326-328 clamps its one row; BerkeleyGW
+    gives every band its own e_n1kq, the mini-app fixes e_n1kq = 6.  With
+    ``levels`` the band energies are drawn from that many distinct values
+    (degenerate bands share a row), else every band gets its own."""
+    rng = np.random.default_rng(seed + 500)
+    if levels is None:
+        e = rng.uniform(2.0, 9.0, nb)
+    else:
+        e = rng.choice(rng.uniform(2.0, 9.0, levels), size=nb)
+    wx = e_lk - e[:, None] + dw * (np.arange(nfreq)[None, :] - 1.0)
+    return np.maximum(wx, 1e-6)

@@ -53,7 +53,7 @@ def assert_close_conditioned(out, seq, dd, tol=TOL, keys=("achtemp", "asxtemp"))
 def problem_of(arrs, nv):
     import pygpp
     nb, ncouls = arrs["aqsmtemp"].shape
-    return pygpp.Problem(nb, nv, ncouls, arrs["I_eps_array"].shape[0], len(arrs["wx_array"]))
+    return pygpp.Problem(nb, nv, ncouls, arrs["I_eps_array"].shape[0], np.shape(arrs["wx_array"])[-1])
 
 
 def run_gpu(ctx, arrs, nv, form):

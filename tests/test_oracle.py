@@ -272,3 +272,60 @@ def test_pole_terms_match_reference_fixtures(oracle):
         _, s1, s2 = oracle.pole_terms(float(wx[i]), complex(wt[i]), complex(ie[i]))
         assert s1 == sch[i] or (np.isnan(s1.real) and np.isnan(sch[i].real)), i
         assert s2 == ssx[i] or (np.isnan(s2.real) and np.isnan(ssx[i].real)), i
+
+
+# ---- band-dependent frequencies (oracle_sigma_wxb) --------------------------
+
+def _one_band(arrs, b, nv):
+    sub = dict(arrs)
+    sub["aqsmtemp"] = arrs["aqsmtemp"][b:b + 1]
+    sub["aqsntemp"] = arrs["aqsntemp"][b:b + 1]
+    sub["wx_array"] = np.ascontiguousarray(arrs["wx_array"][b])
+    return sub, (1 if b < nv else 0)
+
+
+def test_per_band_wx_equal_rows_bitwise(oracle):
+    a = gpp_inputs.seeded_inputs(9, 200, 11, nfreq=4, seed=41)
+    shared = oracle.sigma(a, nvband=3)
+    b = dict(a)
+    b["wx_array"] = np.tile(a["wx_array"], (9, 1))
+    rows = oracle.sigma(b, nvband=3, threads=4)
+    for k in ("achtemp", "asxtemp", "acht_n1_loc"):
+        np.testing.assert_array_equal(shared[k], rows[k], err_msg=k)
+    assert shared["achstemp"] == rows["achstemp"]
+
+
+@pytest.mark.parametrize("use_ref", [False, True])
+def test_per_band_wx_decomposes_by_band(oracle, use_ref):
+    """Each band's acht_n1_loc depends on that band's row only: it must equal,
+    bit for bit, a one-band run with that row as the shared wx_array (run by
+    the oracle, or by the compiled reference harness); the sums over bands
+    equal the sum of the one-band runs to rounding.  Band 1 (occupied) gets
+    negative frequencies, so the ssx cutoff of gppComplex.cpp:58-59 is live."""
+    if use_ref:
+        _ref_or_skip(oracle)
+    nb, nv = 7, 3
+    a = gpp_inputs.seeded_inputs(nb, 160, 10, nfreq=4, seed=42)
+    a["wx_array"] = gpp_inputs.per_band_wx(nb, 4, seed=42)
+    a["wx_array"][1] = [-3.0, -1.0, 0.5, 2.0]
+    assert len({tuple(r) for r in a["wx_array"]}) == nb
+    full = oracle.sigma(a, nvband=nv, threads=3)
+    tot = {k: 0 for k in ("achtemp", "asxtemp", "achstemp")}
+    for b in range(nb):
+        sub, snv = _one_band(a, b, nv)
+        one = oracle.sigma(sub, nvband=snv, use_ref=use_ref)
+        assert full["acht_n1_loc"][b] == one["acht_n1_loc"][0], b
+        for k in tot:
+            tot[k] = tot[k] + one[k]
+    for k in tot:
+        assert rel_err(full[k], tot[k]) <= 1e-13, k
+    # a row-0 or transposed read of wx would change these:
+    shared = oracle.sigma(dict(a, wx_array=np.ascontiguousarray(a["wx_array"][0])), nvband=nv)
+    assert not np.array_equal(shared["acht_n1_loc"][1:], full["acht_n1_loc"][1:])
+
+
+def test_per_band_wx_bad_shape_rejected(oracle):
+    a = gpp_inputs.seeded_inputs(4, 50, 3, seed=43)
+    a["wx_array"] = np.ones((3, 3))
+    with pytest.raises(ValueError):
+        oracle.sigma(a, nvband=1)
