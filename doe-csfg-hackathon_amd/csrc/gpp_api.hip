@@ -9,6 +9,7 @@
 #include <mutex>
 #include <new>
 #include <tuple>
+#include <errno.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -16,6 +17,7 @@
 
 #include "../../include/gpp.h"
 #include "gpp_internal.hpp"
+#include "gpp_io.hpp"
 #include "gpp_rng.hpp"
 
 using gpp::cplx;
@@ -92,6 +94,21 @@ gpp_status check_problem(const gpp_problem *p)
 bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 }  // namespace
+
+namespace gpp {
+
+gpp_status api_error(gpp_status st, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+gpp_status api_check_problem(const gpp_problem *p) { return check_problem(p); }
+
+}  // namespace gpp
 
 // ---------------------------------------------------------------------------
 namespace gpp {
@@ -323,6 +340,115 @@ gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
+    return GPP_OK;
+}
+
+namespace {
+
+// Streams a problem file's sections into the context's device buffers
+// through pinned staging buffers (copy of buffer b overlaps the read of the
+// other); the small sections are kept on the host.
+struct DeviceSink : gpp::io::Sink {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {};
+    bool used[2] = {false, false};
+    void *dev[gpp::io::kSections] = {};
+    std::vector<unsigned char> host[gpp::io::kSections];
+    gpp_status before_fill(int b) override
+    {
+        if (used[b]) HIP_OR_FAIL(hipEventSynchronize(ev[b]), "staging wait");
+        return GPP_OK;
+    }
+    gpp_status piece(int sec, uint64_t off, const void *p, uint64_t n, int) override
+    {
+        if (dev[sec])
+            HIP_OR_FAIL(hipMemcpyAsync(static_cast<char *>(dev[sec]) + off, p, n, hipMemcpyHostToDevice, s),
+                        "H2D file section");
+        else
+            memcpy(host[sec].data() + off, p, n);
+        return GPP_OK;
+    }
+    gpp_status after_chunk(int b) override
+    {
+        HIP_OR_FAIL(hipEventRecord(ev[b], s), "staging event");
+        used[b] = true;
+        return GPP_OK;
+    }
+};
+
+struct PinnedStage {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PinnedStage()
+    {
+        for (int i = 0; i < 2; ++i) {
+            if (buf[i]) (void)hipHostFree(buf[i]);
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+        }
+    }
+};
+
+}  // namespace
+
+gpp_status gpp_upload_file(gpp_ctx *ctx, const char *path, gpp_outputs *expected)
+{
+    using namespace gpp::io;
+    if (!ctx || !path) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(GPP_ERR_IO, "%s: cannot open: %s", path, strerror(errno));
+    struct Closer {
+        FILE *f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    Header h;
+    gpp_status st = read_header(f, path, &h);
+    if (st) return st;
+    if ((st = set_device(ctx))) return st;
+    ctx->have_inputs = false;
+    if ((st = ensure_owned(ctx, &h.prob))) return st;
+    const gpp_problem &p = h.prob;
+    const Layout L = layout_of(p, h.flags);
+
+    PinnedStage stage;
+    const size_t cap = (size_t)32 << 20;
+    for (int i = 0; i < 2; ++i) {
+        HIP_OR_FAIL(hipHostMalloc(&stage.buf[i], cap, hipHostMallocDefault), "pinned staging allocation");
+        HIP_OR_FAIL(hipEventCreateWithFlags(&stage.ev[i], hipEventDisableTiming), "staging event");
+    }
+    DeviceSink sink;
+    sink.s = ctx->stream;
+    sink.ev[0] = stage.ev[0];
+    sink.ev[1] = stage.ev[1];
+    sink.dev[kVcoul] = ctx->vcoul.ptr;
+    sink.dev[kAqsm] = ctx->aqsm.ptr;
+    sink.dev[kAqsn] = ctx->aqsn.ptr;
+    sink.dev[kIeps] = ctx->ieps.ptr;
+    sink.dev[kWtilde] = ctx->wtilde.ptr;
+    for (int sec : {(int)kIdx, (int)kWx, (int)kAchtemp, (int)kAsxtemp, (int)kAcht, (int)kAchs})
+        sink.host[sec].resize(L.bytes[sec]);
+    st = stream_payload(f, path, h, stage.buf, 2, cap, sink);
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "file upload synchronize");
+    if (st) return st;
+
+    const int32_t *idx = reinterpret_cast<const int32_t *>(sink.host[kIdx].data());
+    for (int j = 0; j < p.ngpown; ++j)
+        if (idx[j] < 0) return fail(GPP_ERR_BAD_INDEX, "%s: inv_igp_index[%d] = %d is negative", path, j, idx[j]);
+    const double *wx = reinterpret_cast<const double *>(sink.host[kWx].data());
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->idx.ptr, idx, sizeof(int32_t) * p.ngpown, hipMemcpyHostToDevice, ctx->stream),
+                "H2D inv_igp_index");
+    HIP_OR_FAIL(hipMemcpyAsync(ctx->wx.ptr, wx, sizeof(double) * p.nfreq, hipMemcpyHostToDevice, ctx->stream),
+                "H2D wx_array");
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "file upload synchronize");
+    ctx->prob = p;
+    ctx->have_inputs = true;
+    ctx->per_band = false;
+    if ((h.flags & 1u) && (st = gpp_set_band_frequencies(ctx, wx))) return st;
+    if (expected && (h.flags & 2u)) {
+        if (expected->achtemp) memcpy(expected->achtemp, sink.host[kAchtemp].data(), L.bytes[kAchtemp]);
+        if (expected->asxtemp) memcpy(expected->asxtemp, sink.host[kAsxtemp].data(), L.bytes[kAsxtemp]);
+        if (expected->acht_n1_loc) memcpy(expected->acht_n1_loc, sink.host[kAcht].data(), L.bytes[kAcht]);
+        if (expected->achstemp) memcpy(expected->achstemp, sink.host[kAchs].data(), L.bytes[kAchs]);
+    }
     return GPP_OK;
 }
 

@@ -1,0 +1,93 @@
+// gpp_io.hpp -- the problem file format (.gpp) shared by the host readers and
+// writers of gpp_io.hip and the streaming device upload of gpp_api.hip.
+//
+// Layout (little-endian):
+//   [0, 128)   header
+//       0  char     magic[8] = "GPPSIG01"
+//       8  uint32   version = 1
+//      12  uint32   flags: 1 = wx is [number_bands][nfreq] (band-dependent),
+//                          2 = expected outputs follow the inputs
+//      16  int32    number_bands, nvband, ncouls, ngpown, nfreq
+//      36  int32    0
+//      40  double   occ
+//      48  uint64   payload bytes
+//      56  uint64   payload checksum
+//      64  zeros
+//   [128, 128 + payload)  sections, each starting at a multiple of 64 bytes
+//       (relative to the payload), zero padding between them:
+//       inv_igp_index int32[ngpown], wx double[nfreq or nb*nfreq],
+//       vcoul double[ncouls], aqsmtemp, aqsntemp complex[nb*ncouls],
+//       I_eps_array, wtilde_array complex[ngpown*ncouls]; then, with flag 2,
+//       achtemp[nfreq], asxtemp[nfreq], acht_n1_loc[nb], achstemp[1] (complex;
+//       NaN where the writer had no value).
+// The checksum is FNV-1a over 64-bit words: h = 0xcbf29ce484222325, then
+// h = (h ^ w) * 0x100000001b3 (mod 2^64) for every little-endian word w of
+// the payload, padding included.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/gpp.h"
+
+namespace gpp {
+
+// Error reporting of the C ABI (thread-local message of gpp_last_error).
+gpp_status api_error(gpp_status st, const char *fmt, ...);
+gpp_status api_check_problem(const gpp_problem *p);
+
+namespace io {
+
+constexpr char kMagic[8] = {'G', 'P', 'P', 'S', 'I', 'G', '0', '1'};
+constexpr uint32_t kVersion = 1;
+constexpr size_t kHeaderBytes = 128;
+constexpr uint64_t kHashInit = 0xcbf29ce484222325ull;
+constexpr uint64_t kHashPrime = 0x100000001b3ull;
+
+enum Section { kIdx, kWx, kVcoul, kAqsm, kAqsn, kIeps, kWtilde, kAchtemp, kAsxtemp, kAcht, kAchs, kSections };
+
+struct Header {
+    gpp_problem prob;
+    uint32_t flags;
+    uint64_t payload;
+    uint64_t checksum;
+};
+
+struct Layout {
+    uint64_t off[kSections];    // offset in the payload (0 for an absent section)
+    uint64_t bytes[kSections];  // 0 for an absent section
+    uint64_t payload;
+};
+
+Layout layout_of(const gpp_problem &p, uint32_t flags);
+
+inline uint64_t hash_words(uint64_t h, const void *p, size_t n)  // n % 8 == 0
+{
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; i += 8) {
+        uint64_t w;
+        __builtin_memcpy(&w, b + i, 8);
+        h = (h ^ w) * kHashPrime;
+    }
+    return h;
+}
+
+// Reads and validates the header of an open file (positioned anywhere).
+gpp_status read_header(void *file, const char *path, Header *h);
+
+// Streams the payload of an open file through buf (cap bytes, a multiple of
+// 64): for every piece of a section read, calls
+// sink(section, offset_in_section, ptr, bytes); then checks the checksum.
+// before_fill(i) is called before chunk i is read into buffer `bufs[i % nbufs]`
+// so that the caller can wait for its earlier use.
+struct Sink {
+    virtual ~Sink() = default;
+    virtual gpp_status before_fill(int buf) { (void)buf; return GPP_OK; }
+    virtual gpp_status piece(int section, uint64_t off, const void *ptr, uint64_t bytes, int buf) = 0;
+    virtual gpp_status after_chunk(int buf) { (void)buf; return GPP_OK; }
+};
+gpp_status stream_payload(void *file, const char *path, const Header &h, void *const *bufs, int nbufs, size_t cap,
+                          Sink &sink);
+
+}  // namespace io
+}  // namespace gpp

@@ -63,6 +63,10 @@ EXPORTED_SYMBOLS = (
     "gpp_generate_seeded",
     "gpp_sigma",
     "gpp_set_band_frequencies",
+    "gpp_file_write",
+    "gpp_file_info",
+    "gpp_file_read",
+    "gpp_upload_file",
     "gpp_ctx_band_buckets",
     "gpp_ctx_last_form",
     "gpp_ctx_set_option",
@@ -104,6 +108,20 @@ class GppOutputs(ctypes.Structure):
     ]
 
 
+class GppFileHeader(ctypes.Structure):
+    _fields_ = [
+        ("problem", GppProblem),
+        ("flags", ctypes.c_uint32),
+        ("payload_bytes", ctypes.c_uint64),
+        ("checksum", ctypes.c_uint64),
+    ]
+
+
+GPP_FILE_WX_PER_BAND = 1
+GPP_FILE_HAS_EXPECTED = 2
+GPP_ERR_IO = 9
+
+
 class GppDeviceInputs(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("inv_igp_index", "wx_array", "vcoul", "aqsmtemp", "aqsntemp",
                                               "I_eps_array", "wtilde_array")]
@@ -141,6 +159,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_generate_seeded.argtypes = [vp, P(GppProblem), ctypes.c_uint64, dbl, dbl]
     lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
     lib.gpp_set_band_frequencies.argtypes = [vp, vp]
+    lib.gpp_file_write.argtypes = [ctypes.c_char_p, P(GppProblem), vp, vp, i32] + [vp] * 5 + [P(GppOutputs)]
+    lib.gpp_file_info.argtypes = [ctypes.c_char_p, P(GppFileHeader)]
+    lib.gpp_file_read.argtypes = [ctypes.c_char_p] + [vp] * 7 + [P(GppOutputs)]
+    lib.gpp_upload_file.argtypes = [vp, ctypes.c_char_p, P(GppOutputs)]
     lib.gpp_ctx_band_buckets.argtypes = [vp, P(i32)]
     lib.gpp_ctx_last_form.argtypes = [vp, P(i32)]
     lib.gpp_ctx_last_timing.argtypes = [vp, P(dbl), P(dbl), i32, P(i32)]
@@ -231,6 +253,59 @@ def fill_seeded(p: Problem, seed: int, wx_center: float = 4.0, dw: float = 1.0) 
     return arrs
 
 
+_OUT_KEYS = ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")
+
+
+def _expected_arrays(p: Problem) -> Dict[str, np.ndarray]:
+    return {"achtemp": np.zeros(p.nfreq, np.complex128), "asxtemp": np.zeros(p.nfreq, np.complex128),
+            "acht_n1_loc": np.zeros(p.number_bands, np.complex128), "achstemp": np.zeros(1, np.complex128)}
+
+
+def write_problem(path: str, p: Problem, arrs: Dict[str, np.ndarray],
+                  expected: Optional[Dict[str, np.ndarray]] = None) -> None:
+    """Writes a problem file (.gpp).  A 2-D wx_array is stored as
+    band-dependent frequencies; expected outputs are optional."""
+    wx = np.ascontiguousarray(arrs["wx_array"], np.float64)
+    a = {k: _contig(arrs[k], np.complex128) for k in ("aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array")}
+    a["inv_igp_index"] = _contig(arrs["inv_igp_index"], np.int32)
+    a["vcoul"] = _contig(arrs["vcoul"], np.float64)
+    a["wx_array"] = wx
+    o = None
+    if expected is not None:
+        e = {k: (None if expected.get(k) is None else np.ascontiguousarray(np.atleast_1d(expected[k]), np.complex128))
+             for k in _OUT_KEYS}
+        o = GppOutputs(*[_ptr(e[k]) for k in _OUT_KEYS])
+    prob = p.c()
+    _check(load().gpp_file_write(os.fsencode(path), ctypes.byref(prob), _ptr(a["inv_igp_index"]), _ptr(wx),
+                                 1 if wx.ndim == 2 else 0, _ptr(a["vcoul"]),
+                                 *[_ptr(a[k]) for k in ("aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array")],
+                                 ctypes.byref(o) if o is not None else None))
+
+
+def file_info(path: str) -> Dict[str, object]:
+    h = GppFileHeader()
+    _check(load().gpp_file_info(os.fsencode(path), ctypes.byref(h)))
+    q = h.problem
+    return {"problem": Problem(q.number_bands, q.nvband, q.ncouls, q.ngpown, q.nfreq, q.occ),
+            "flags": h.flags, "payload_bytes": h.payload_bytes, "checksum": h.checksum}
+
+
+def read_problem(path: str):
+    """(Problem, inputs, expected outputs or None) from a problem file."""
+    info = file_info(path)
+    p = info["problem"]
+    arrs = empty_inputs(p)
+    if info["flags"] & GPP_FILE_WX_PER_BAND:
+        arrs["wx_array"] = np.zeros((p.number_bands, p.nfreq), np.float64)
+    exp = _expected_arrays(p) if info["flags"] & GPP_FILE_HAS_EXPECTED else None
+    o = GppOutputs(*[_ptr(exp[k]) for k in _OUT_KEYS]) if exp is not None else None
+    _check(load().gpp_file_read(os.fsencode(path), *[_ptr(arrs[k]) for k in _INPUT_ORDER],
+                                ctypes.byref(o) if o is not None else None))
+    if exp is not None:
+        exp["achstemp"] = exp["achstemp"][0]
+    return p, arrs, exp
+
+
 def _contig(a: np.ndarray, dtype) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=dtype)
 
@@ -279,6 +354,20 @@ class Context:
         self.problem = p
         if wx.ndim == 2:
             self.set_band_frequencies(wx)
+
+    def upload_file(self, path: str):
+        """Streams a problem file to the device; returns the file's expected
+        outputs, or None when it holds none."""
+        info = file_info(path)
+        p = info["problem"]
+        exp = _expected_arrays(p)
+        o = GppOutputs(*[_ptr(exp[k]) for k in _OUT_KEYS])
+        _check(load().gpp_upload_file(self._h, os.fsencode(path), ctypes.byref(o)))
+        self.problem = p
+        if not info["flags"] & GPP_FILE_HAS_EXPECTED:
+            return None
+        exp["achstemp"] = exp["achstemp"][0]
+        return exp
 
     def set_band_frequencies(self, wx_bands: Optional[np.ndarray]) -> None:
         """Band n1 uses wx_bands[n1] ([number_bands][nfreq]); None restores

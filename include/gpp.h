@@ -49,7 +49,8 @@ typedef enum {
     GPP_ERR_LAUNCH = 5,           /* kernel launch or execution error */
     GPP_ERR_UNSUPPORTED = 6,      /* form not available for this problem */
     GPP_ERR_NO_INPUTS = 7,        /* gpp_sigma before gpp_upload / gpp_bind_device */
-    GPP_ERR_NONFINITE = 8         /* reserved: non-finite output reported by a check */
+    GPP_ERR_NONFINITE = 8,        /* reserved: non-finite output reported by a check */
+    GPP_ERR_IO = 9                /* problem file: open/read/write failure, bad header, truncation, checksum */
 } gpp_status;
 
 /* One complex number, {re, im}; Complex.h:15-19. */
@@ -241,6 +242,59 @@ gpp_status gpp_ctx_band_buckets(const gpp_ctx *ctx, int *nbuckets);
 
 /* The form the last gpp_sigma on this context actually ran. */
 gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form);
+
+/* ---- problem files (.gpp) ----------------------------------------------------
+ *
+ * A problem file holds one problem's inputs and, optionally, its expected
+ * outputs, so that inputs produced elsewhere (BerkeleyGW's eps / wtilde /
+ * vcoul, or golden vectors) go through the same ABI.  Layout, little-endian:
+ * a 128-byte header (magic "GPPSIG01", version 1, flags, the gpp_problem
+ * sizes, occ, payload size, payload checksum), then the sections
+ * inv_igp_index int32[ngpown], wx double[nfreq] (flag GPP_FILE_WX_PER_BAND:
+ * [number_bands][nfreq]), vcoul double[ncouls], aqsmtemp, aqsntemp
+ * complex[number_bands][ncouls], I_eps_array, wtilde_array
+ * complex[ngpown][ncouls] and (flag GPP_FILE_HAS_EXPECTED) achtemp[nfreq],
+ * asxtemp[nfreq], acht_n1_loc[number_bands], achstemp[1], each section
+ * starting at a multiple of 64 bytes of the payload with zero padding
+ * between.  Checksum: FNV-1a over the payload's 64-bit little-endian words,
+ * h = 0xcbf29ce484222325, h = (h ^ w) * 0x100000001b3 mod 2^64.
+ */
+#define GPP_FILE_WX_PER_BAND 1u
+#define GPP_FILE_HAS_EXPECTED 2u
+
+typedef struct {
+    gpp_problem problem;
+    uint32_t flags;          /* GPP_FILE_* */
+    uint64_t payload_bytes;  /* bytes after the 128-byte header */
+    uint64_t checksum;
+} gpp_file_header;
+
+/* Writes a problem file.  wx holds nfreq values, or number_bands x nfreq
+ * when wx_per_band is non-zero.  expected may be NULL; if given, achtemp and
+ * asxtemp are required and a NULL acht_n1_loc / achstemp is stored as NaN. */
+gpp_status gpp_file_write(const char *path, const gpp_problem *prob, const int32_t *inv_igp_index,
+                          const double *wx, int wx_per_band, const double *vcoul,
+                          const gpp_complex *aqsmtemp, const gpp_complex *aqsntemp,
+                          const gpp_complex *I_eps_array, const gpp_complex *wtilde_array,
+                          const gpp_outputs *expected);
+
+/* Reads and validates a file's header (the payload is not read). */
+gpp_status gpp_file_info(const char *path, gpp_file_header *info);
+
+/* Reads a whole file into caller-allocated host arrays sized from
+ * gpp_file_info (wx: number_bands x nfreq with GPP_FILE_WX_PER_BAND) and
+ * verifies the checksum.  expected may be NULL; it is filled only when the
+ * file holds expected outputs (its NULL members are skipped). */
+gpp_status gpp_file_read(const char *path, int32_t *inv_igp_index, double *wx, double *vcoul,
+                         gpp_complex *aqsmtemp, gpp_complex *aqsntemp, gpp_complex *I_eps_array,
+                         gpp_complex *wtilde_array, gpp_outputs *expected);
+
+/* Uploads a problem file into the context as gpp_upload does, streaming the
+ * large sections from disk to the device through two pinned 32 MB staging
+ * buffers (host memory use does not grow with the problem), and sets the
+ * band frequencies when the file holds them.  The checksum is verified
+ * before the context accepts the inputs.  expected as for gpp_file_read. */
+gpp_status gpp_upload_file(gpp_ctx *ctx, const char *path, gpp_outputs *expected);
 
 /* Sets / reads a context option (gpp_option). */
 gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value);

@@ -9,8 +9,13 @@ feeds it the inputs below and stores inputs and outputs:
                          near-pole, several nfreq / nvband / index clamps)
   ref_pole_vectors.npz   ssxt_scht_solver's sch/ssx for a grid of
                          (wx, wtilde, I_eps) covering all three branches
+  ref_near_pole_16x3.gpp one near-pole problem as a problem file (the
+                         library's .gpp format) with the reference's outputs
+                         as its expected outputs
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [gpp]
+
+("gpp" writes only the problem file.)
 """
 import os
 import sys
@@ -64,9 +69,24 @@ def pole_grid():
     return np.array(wx), np.array(wt), np.array(ie)
 
 
+GPP_FIXTURE = "ref_near_pole_16x3.gpp"
+
+
+def write_problem_fixture():
+    sys.path.insert(0, os.path.join(REPO, "doe-csfg-hackathon_amd"))
+    import pygpp
+    a = gpp_inputs.near_pole_inputs(16, 96, 12, seed=9)
+    r = gpp_oracle.sigma(a, nvband=3, use_ref=True)
+    p = pygpp.Problem(16, 3, 96, 12, 3)
+    pygpp.write_problem(os.path.join(HERE, GPP_FIXTURE), p, a, expected=r)
+
+
 def main():
     if not gpp_oracle.ref_available():
         sys.exit("oracle/_ref/libgppref.so missing: run make -C oracle ref (needs the reference sources)")
+    write_problem_fixture()
+    if sys.argv[1:] == ["gpp"]:
+        return
     out = {}
     for name, a, nv in cases():
         r = gpp_oracle.sigma(a, nvband=nv, use_ref=True)

@@ -118,3 +118,8 @@ def test_driver_usage_error_exit_status(gpplib):
     assert r.returncode == 2 and "correct form of input" in r.stdout
     r = subprocess.run([drv, "4", "1", "10", "2", "--form", "bogus"], capture_output=True, text=True)
     assert r.returncode == 2
+    # --load takes no positional arguments; a missing file is a runtime error (1)
+    r = subprocess.run([drv, "4", "1", "10", "2", "--load", "x.gpp"], capture_output=True, text=True)
+    assert r.returncode == 2
+    r = subprocess.run([drv, "--load", os.path.join(REPO, "no_such_file.gpp")], capture_output=True, text=True)
+    assert r.returncode == 1 and "gpp_file_info" in r.stderr
