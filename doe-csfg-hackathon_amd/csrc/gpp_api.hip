@@ -182,6 +182,7 @@ struct gpp_ctx {
     int gram_tile = 0;
     int red_chunk = 8192;
     int red_fc = 8;
+    int red_fpb = 0;
     int num_cus = 256;
     int max_buckets = 32;
     // band-dependent frequencies (gpp_set_band_frequencies)
@@ -243,6 +244,7 @@ gpp::GramKind gram_kind(const gpp_ctx *ctx, gpp_form form)
     k.tile = ctx->gram_tile;
     k.red_chunk = ctx->red_chunk;
     k.red_fc = ctx->red_fc;
+    k.red_fpb = ctx->red_fpb;
     return k;
 }
 
@@ -726,6 +728,11 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         if (value != 4 && value != 8) return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs must be 4 or 8, got %d", value);
         ctx->red_fc = value;
         return GPP_OK;
+    case GPP_OPT_REDUCE_FREQS_PER_BLOCK:
+        if (value < 0 || value > 4096)
+            return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs per block must be 0..4096, got %d", value);
+        ctx->red_fpb = value;
+        return GPP_OK;
     case GPP_OPT_MAX_BUCKETS:
         if (value < 1 || value > (1 << 20))
             return fail(GPP_ERR_INVALID_ARGUMENT, "max buckets must be in [1, 2^20], got %d", value);
@@ -745,6 +752,7 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
     case GPP_OPT_REDUCE_CHUNK: *value = ctx->red_chunk; return GPP_OK;
     case GPP_OPT_REDUCE_FREQS: *value = ctx->red_fc; return GPP_OK;
     case GPP_OPT_MAX_BUCKETS: *value = ctx->max_buckets; return GPP_OK;
+    case GPP_OPT_REDUCE_FREQS_PER_BLOCK: *value = ctx->red_fpb; return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
 }

@@ -93,7 +93,9 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__res
         ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
         ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
-        pole_reduce_item<FC>(ra, it.j, it.split, it.iw0, scratch);
+        ra.fpb = FC;
+        pole_reduce_item<FC>(ra, it.j, it.split, it.iw0, it.iw0 + FC < b.nfreq ? it.iw0 + FC : b.nfreq,
+                             scratch);
     }
 }
 
@@ -101,7 +103,9 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_final(const BInst *__rest
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const BInst &b = insts[blockIdx.x];
-    final_block(b.col_ach, b.col_asx, b.part_achs, b.ngpown * b.nsplit, b.nfreq, b.nb, b.nv, b.out, scratch);
+    if ((int)blockIdx.y > b.nfreq) return;
+    final_block(b.col_ach, b.col_asx, b.part_achs, b.ngpown * b.nsplit, b.nfreq, b.nb, b.nv, b.out, scratch,
+                blockIdx.y);
 }
 
 __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -116,6 +120,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
 {
     BatchPlan pl;
     pl.n = (int)probs.size();
+    for (const DevProblem &p : probs) pl.max_nfreq = std::max(pl.max_nfreq, p.nfreq);
     pl.kind = k;
     size_t off = 0;
     pl.inst_off.resize(pl.n);
@@ -244,7 +249,8 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         else
             hipLaunchKernelGGL((k_batch_reduce<8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, pl.kind.red_chunk);
     }
-    if (which & 8) hipLaunchKernelGGL(k_batch_final, dim3(pl.n), dim3(kRedThreads), 0, s, insts);
+    if (which & 8)
+        hipLaunchKernelGGL(k_batch_final, dim3(pl.n, pl.max_nfreq + 1), dim3(kRedThreads), 0, s, insts);
     return hipGetLastError();
 }
 

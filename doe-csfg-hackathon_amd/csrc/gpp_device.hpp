@@ -74,10 +74,26 @@ __device__ __forceinline__ double rcp_refined(double d)
     return r;
 }
 
+// 1/d with one Newton-Raphson step: about 2^-45 relative (the estimate is
+// good to about 2^-23).  For values that feed sums, not branch decisions.
+__device__ __forceinline__ double rcp_1nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 // Product a*b with FMA contraction allowed (fast paths only).
 __device__ __forceinline__ cplx cmul_fast(cplx a, cplx b)
 {
     return cplx{__builtin_fma(a.re, b.re, -a.im * b.im), __builtin_fma(a.re, b.im, a.im * b.re)};
+}
+
+// acc + a*b with four FMAs.
+__device__ __forceinline__ cplx cfma(cplx a, cplx b, cplx acc)
+{
+    return cplx{__builtin_fma(a.re, b.re, __builtin_fma(-a.im, b.im, acc.re)),
+                __builtin_fma(a.re, b.im, __builtin_fma(a.im, b.re, acc.im))};
 }
 
 // Wave-wide sum by xor butterfly.  Every lane ends with the same, bitwise

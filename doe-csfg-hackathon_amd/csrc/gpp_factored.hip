@@ -108,7 +108,9 @@ template <int FC>
 __global__ __launch_bounds__(kRedThreads) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
-    pole_reduce_item<FC>(a, blockIdx.x, blockIdx.y, blockIdx.z * FC, scratch);
+    const int iw_begin = blockIdx.z * a.fpb;
+    const int iw_end = iw_begin + a.fpb < a.nfreq ? iw_begin + a.fpb : a.nfreq;
+    pole_reduce_item<FC>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach, const cplx *col_asx,
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach,
                                                             int nv, cplx *out)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
-    final_block(col_ach, col_asx, part_achs, ncols, nfreq, nb, nv, out, scratch);
+    final_block(col_ach, col_asx, part_achs, ncols, nfreq, nb, nv, out, scratch, blockIdx.x);
 }
 
 // Grid (column tiles, j tiles, G-vector splits): the column tiles that share
@@ -162,9 +164,10 @@ void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t 
 }
 
 template <int FC>
-void launch_reduce_fc(const ReduceArgs &a, hipStream_t s)
+void launch_reduce_fc(ReduceArgs a, hipStream_t s)
 {
-    dim3 grid(a.ngpown, a.nsplit, (a.nfreq + FC - 1) / FC);
+    if (a.fpb < FC) a.fpb = FC;
+    dim3 grid(a.ngpown, a.nsplit, (a.nfreq + a.fpb - 1) / a.fpb);
     hipLaunchKernelGGL((k_pole_reduce<FC>), grid, dim3(kRedThreads), 0, s, a);
 }
 
@@ -242,12 +245,13 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     ra.vcoul = p.vcoul; ra.wx = p.wx; ra.nv = p.nv; ra.ncouls = p.ncouls; ra.ngpown = p.ngpown;
     ra.nfreq = p.nfreq; ra.occ = p.occ; ra.col_ach = col_ach; ra.col_asx = col_asx;
     ra.nsplit = L.nsplit; ra.chunk = kind.red_chunk;
+    ra.fpb = kind.red_fpb > 0 ? kind.red_fpb : p.nfreq;
     ra.p2 = kind.acht ? reinterpret_cast<cplx *>(w + L.p2) : nullptr;
     ra.br1 = kind.acht && p.nv > 0 ? reinterpret_cast<uint8_t *>(w + L.br1) : nullptr;
     ra.acht_iw = p.nfreq > 2 ? 2 : p.nfreq - 1;  // sch_array[2], gppComplex.cpp:381
     GPP_TRY(launch_pole_reduce(ra, kind.red_fc, s));
     if (timer) timer->mark(s, "factored_pole_reduce");
-    hipLaunchKernelGGL(k_fact_final, dim3(1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs,
+    hipLaunchKernelGGL(k_fact_final, dim3(p.nfreq + 1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs,
                        p.ngpown * L.nsplit, p.nfreq, p.nb, p.nv, res.base);
     GPP_TRY(hipGetLastError());
     if (timer) timer->mark(s, "factored_final");

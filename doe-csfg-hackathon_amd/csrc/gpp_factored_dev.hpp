@@ -151,6 +151,7 @@ struct ReduceArgs {
     const int32_t *idx;
     const double *vcoul, *wx;
     int nv, ncouls, ngpown, nfreq, nsplit, chunk;
+    int fpb;  // frequencies per block (single-instance grid: z = ceil(nfreq / fpb))
     double occ;
     cplx *col_ach, *col_asx;  // [ngpown][nsplit][nfreq]
     // Optional (acht_n1_loc): the pole factor of frequency acht_iw, p2 =
@@ -163,43 +164,58 @@ struct ReduceArgs {
 };
 
 // Stage 3: pole terms of column j over G-vectors [split*chunk, +chunk), for
-// frequencies [iw0, iw0+FC).  Per element and frequency: delw =
-// wt*conj(wdiff)*rden and P = delw*ie in the reference's order
-// (gppComplex.cpp:139-146, no contraction; rden is the refined reciprocal,
+// frequencies [iw0, iw0+FC).  Per element and frequency: wdiff, |wdiff|^2,
+// delw = wt*conj(wdiff)*rden and |delw|^2 in the reference's order
+// (gppComplex.cpp:139-144, no contraction; rden is the refined reciprocal,
 // equal to the IEEE quotient), so the occupied branch tests (:35, :43) see
-// the reference's values; sch = P in branch 1 (:37) shares the unoccupied
-// product, ssx follows :38-51.  Block-wide fixed-order sums; writes the
-// column partials (0.5*sum)*vcoul and sum*(occ*vcoul).  scratch: 4 entries.
+// the reference's values.  P = delw*ie, the branch-2 ssx in its reduced form
+// -(I_eps wt / 2) / (wx + wt) and the accumulations use FMA.  sch = P in branch 1 (:37) shares the
+// unoccupied product.  Block-wide fixed-order sums; writes the column
+// partials (0.5*sum)*vcoul and sum*(occ*vcoul).  scratch: 4 entries.
 template <int FC>
-__device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw0, cplx *scratch)
+__device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw_begin, int iw_end,
+                                                 cplx *scratch)
 {
 #pragma clang fp contract(off)
     const int ncouls = a.ncouls;
     const int nv = a.nv;
+
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
     const double limitone = 1.0 / (kTo1 * 4.0);
-    double wxv[FC], wx2[FC], rwx4[FC];
+    // Frequencies in passes of FC; the passes after the first re-read this
+    // block's G-vector range, recently fetched, from the caches.
+    for (int iw0 = iw_begin; iw0 < iw_end; iw0 += FC) {
+    double wxv[FC], c1[FC];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
-        wxv[f] = (iw0 + f < a.nfreq) ? a.wx[iw0 + f] : 1.0;
-        wx2[f] = wxv[f] * wxv[f];
-        rwx4[f] = 1.00 / (wx2[f] * wx2[f]);
+        wxv[f] = (iw0 + f < iw_end) ? a.wx[iw0 + f] : 1.0;
+        const double wx2 = wxv[f] * wxv[f];
+        c1[f] = wx2 * (1.00 / (wx2 * wx2));
     }
     cplx sach[FC], sasx[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) sach[f] = sasx[f] = czero();
+
+    bool any_neg = false;  // the ssx cutoff of :58-59 applies only at negative frequencies
+#pragma unroll
+    for (int f = 0; f < FC; ++f) any_neg = any_neg || wxv[f] < 0.00;
 
     const size_t row = (size_t)j * ncouls;
     for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
         const cplx wt = a.wtilde[row + ig];
         const cplx ie = a.ieps[row + ig];
         const cplx mu = a.mu[row + ig];
-        cplx mo = czero(), om2 = czero();
+        cplx mo = czero(), om2 = czero(), mum = mu, qh = czero();
+        double ie_cut = 0.0, wti2 = 0.0;
         if (nv > 0) {
             for (int n1 = 0; n1 < nv; ++n1)  // occupied bands, in band order
                 mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
             om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
+            mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
+            qh = cscale(cmul_fast(ie, wt), 0.5);
+            wti2 = wt.im * wt.im;
+            if (any_neg) ie_cut = cabs_(ie) * kSexcut;
         }
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
@@ -207,7 +223,7 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             const double wdiffr = cmul(wdiff, cconj(wdiff)).re;
             const double rden = rcp_refined(wdiffr);
             const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);
-            const cplx p = cmul(delw, ie);
+            const cplx p = cmul_fast(delw, ie);
             cplx m = mu;
             bool branch1 = false;
             if (nv > 0) {
@@ -215,20 +231,24 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 cplx ssx = czero();
                 if ((wdiffr > 0.25) && (delwr < limitone)) {
                     branch1 = true;
-                    m = cadd(mu, mo);  // sch = P for the occupied bands
-                    ssx = cscale2(om2, wx2[f], rwx4[f]);
+                    m = mum;
+                    ssx = cscale(om2, c1[f]);
                 } else if (delwr > kTo1) {
-                    const cplx cden = cscale(cmul(csquare(wt), cdplus(0.50, delw)), 4.00);
-                    const double rd = rcp_refined(cmul(cden, cconj(cden)).re);
-                    ssx = cmul(cmul(cplx{-om2.re, -om2.im}, cconj(cden)), cscale(delw, rd));
+                    // :46-50 with cden = 4 wt^2 (0.5 + delw), Omega2 = wt^2 I_eps
+                    // and delw = wt / (wx - wt):
+                    //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
+                    //                                     = -(I_eps wt / 2) / (wx + wt),
+                    // a pole at -wt, far from cancellation for wx > 0.
+                    const double s = wxv[f] + wt.re;
+                    const double rr = rcp_1nr(__builtin_fma(s, s, wti2));
+                    ssx = cplx{-__builtin_fma(qh.re, s, qh.im * wt.im) * rr,
+                               -__builtin_fma(qh.im, s, -(qh.re * wt.im)) * rr};
                 }
-                if (wxv[f] < 0.00 && cabs_(ssx) > cabs_(ie) * kSexcut) ssx = czero();
-                const cplx x = cmul_fast(ssx, mo);
-                sasx[f] = cadd(sasx[f], x);
+                if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
+                sasx[f] = cfma(ssx, mo, sasx[f]);
             }
-            const cplx t = cmul_fast(p, m);
-            sach[f] = cadd(sach[f], t);
-            if (a.p2 && iw0 + f == a.acht_iw) {
+            sach[f] = cfma(p, m, sach[f]);
+            if (a.p2 && iw0 + f == a.acht_iw && iw0 + f < iw_end) {
                 a.p2[row + ig] = p;
                 if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
             }
@@ -239,12 +259,13 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     for (int f = 0; f < FC; ++f) {
         const cplx s = block_sum(sach[f], scratch);
         const cplx x = block_sum(sasx[f], scratch);
-        if (threadIdx.x == 0 && iw0 + f < a.nfreq) {
+        if (threadIdx.x == 0 && iw0 + f < iw_end) {
             // (0.5 * sum) * vcoul and sum * (occ * vcoul), gppComplex.cpp:361-363, :379
             const size_t o = ((size_t)j * a.nsplit + split) * a.nfreq + iw0 + f;
             a.col_ach[o] = cscale(cscale(s, 0.5), vc);
             a.col_asx[o] = cscale(x, a.occ * vc);
         }
+    }
     }
 }
 
@@ -399,13 +420,15 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const
     }
 }
 
-// Stage 4 (one block): achtemp/asxtemp as fixed-order sums over the
-// (column, split) partials, acht_n1_loc set to NaN (not produced by this
-// form), achstemp as the fixed-order sum of the per-band partials.
+// Stage 4: block `part` < nfreq forms achtemp[part] / asxtemp[part] as
+// fixed-order sums over the (column, split) partials; block part == nfreq
+// sets acht_n1_loc to NaN (the factored forms write it in a later stage when
+// it is requested) and sums the per-band achstemp partials in order.
 __device__ __forceinline__ void final_block(const cplx *col_ach, const cplx *col_asx, const cplx *part_achs, int ncols,
-                                            int nfreq, int nb, int nv, cplx *out, cplx *scratch)
+                                            int nfreq, int nb, int nv, cplx *out, cplx *scratch, int part)
 {
-    for (int iw = 0; iw < nfreq; ++iw) {
+    if (part < nfreq) {
+        const int iw = part;
         cplx s = czero(), x = czero();
         for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
             s = cadd(s, col_ach[(size_t)c * nfreq + iw]);
@@ -417,6 +440,7 @@ __device__ __forceinline__ void final_block(const cplx *col_ach, const cplx *col
             out[iw] = s;
             out[nfreq + iw] = nv > 0 ? x : czero();
         }
+        return;
     }
     const double nan = __builtin_nan("");
     for (int n1 = threadIdx.x; n1 < nb; n1 += blockDim.x) out[2 * nfreq + n1] = cplx{nan, nan};

@@ -100,6 +100,7 @@ struct GramKind {
     int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
     int red_chunk = 8192;  // G-vectors per pole-reduce block
     int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
+    int red_fpb = 0;       // frequencies per pole-reduce block (0: all, passes looped in the block)
     bool acht = false;     // also produce acht_n1_loc (a second product, R * aqsn^T)
 };
 constexpr int kGramTiles = 4;
@@ -146,6 +147,7 @@ struct BatchPlan {
     };
     int n = 0;
     int fc = 8;
+    int max_nfreq = 0;
     GramKind kind;
     std::vector<InstOff> inst_off;
     std::vector<GramItem> gram_items;

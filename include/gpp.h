@@ -118,7 +118,12 @@ typedef enum {
      * groups of bands sharing one frequency row that the factored forms
      * factor group by group; beyond it they run the direct form (default
      * 32). */
-    GPP_OPT_MAX_BUCKETS = 6
+    GPP_OPT_MAX_BUCKETS = 6,
+    /* Frequencies one pole-reduce block evaluates, in passes of
+     * GPP_OPT_REDUCE_FREQS, re-reading its G-vector range from the caches
+     * between passes: 0 = all nfreq (default); k > 0 splits the frequencies
+     * over ceil(nfreq / k) blocks per range. */
+    GPP_OPT_REDUCE_FREQS_PER_BLOCK = 7
 } gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).
