@@ -134,33 +134,38 @@ void Workspace::release()
 
 hipError_t StageTimer::init()
 {
-    for (int i = 0; i <= kMax; ++i) {
-        hipError_t e = hipEventCreate(&ev[i]);
-        if (e != hipSuccess) return e;
-    }
+    for (int k = 0; k < kRing; ++k)
+        for (int i = 0; i <= kMax; ++i) {
+            hipError_t e = hipEventCreate(&slot[k].ev[i]);
+            if (e != hipSuccess) return e;
+        }
     ready = true;
     return hipSuccess;
 }
 
 void StageTimer::destroy()
 {
-    if (!ready) return;
-    for (int i = 0; i <= kMax; ++i) (void)hipEventDestroy(ev[i]);
+    for (int k = 0; k < kRing; ++k)
+        for (int i = 0; i <= kMax; ++i)
+            if (slot[k].ev[i]) (void)hipEventDestroy(slot[k].ev[i]);
     ready = false;
 }
 
 void StageTimer::begin(hipStream_t s)
 {
-    n = 0;
-    if (ready) (void)hipEventRecord(ev[0], s);
+    if (filled > 0) cur = (cur + 1) % kRing;
+    if (filled < kRing) ++filled;
+    slot[cur].n = 0;
+    if (ready) (void)hipEventRecord(slot[cur].ev[0], s);
 }
 
 void StageTimer::mark(hipStream_t s, const char *nm)
 {
-    if (!ready || n >= kMax) return;
-    name[n] = nm;
-    (void)hipEventRecord(ev[n + 1], s);
-    ++n;
+    Slot &sl = slot[cur];
+    if (!ready || sl.n >= kMax) return;
+    sl.name[sl.n] = nm;
+    (void)hipEventRecord(sl.ev[sl.n + 1], s);
+    ++sl.n;
 }
 
 }  // namespace gpp
@@ -190,7 +195,8 @@ struct gpp_ctx {
     DevBuf wxb;                     // [nb][nfreq]
     gpp::BucketPlan bplan;          // bands grouped by identical rows
     gpp_form last_form = GPP_FORM_DIRECT;
-    bool have_last_form = false;
+    bool have_last_form = false;  // an evaluation was enqueued on the current inputs
+    bool last_acht = false;       // ... and it computed acht_n1_loc
     // autotune results: (form, nb, nv, ncouls, ngpown, nfreq) -> (tile, chunk, fc)
     std::map<std::tuple<int, int, int, int, int, int>, std::tuple<int, int, int>> tuned;
 };
@@ -342,6 +348,7 @@ gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
+    ctx->have_last_form = false;
     return GPP_OK;
 }
 
@@ -444,6 +451,7 @@ gpp_status gpp_upload_file(gpp_ctx *ctx, const char *path, gpp_outputs *expected
     ctx->prob = p;
     ctx->have_inputs = true;
     ctx->per_band = false;
+    ctx->have_last_form = false;
     if ((h.flags & 1u) && (st = gpp_set_band_frequencies(ctx, wx))) return st;
     if (expected && (h.flags & 2u)) {
         if (expected->achtemp) memcpy(expected->achtemp, sink.host[kAchtemp].data(), L.bytes[kAchtemp]);
@@ -482,6 +490,7 @@ gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t 
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
+    ctx->have_last_form = false;
     return GPP_OK;
 }
 
@@ -593,6 +602,7 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
+    ctx->have_last_form = false;
     return GPP_OK;
 }
 
@@ -634,11 +644,14 @@ gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form)
     return GPP_OK;
 }
 
-gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
+}  // extern "C"
+
+namespace {
+
+// Enqueues one evaluation on the context's stream; the results stay in the
+// context's device result buffer.
+gpp_status sigma_enqueue(gpp_ctx *ctx, gpp_form form, bool acht)
 {
-    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    if (!out || !out->achtemp || !out->asxtemp)
-        return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
     if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
     if (form < GPP_FORM_DIRECT || form > GPP_FORM_FACTORED_OZAKI_I8)
         return fail(GPP_ERR_INVALID_ARGUMENT, "unknown form %d", (int)form);
@@ -662,16 +675,15 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
             else if (factored) form = GPP_FORM_DIRECT;
         }
     }
+    const bool direct = form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT;
     size_t need = 0;
     gpp::GramKind kind;
-    switch (form) {
-    case GPP_FORM_DIRECT:
-    case GPP_FORM_DIRECT_STRICT: need = gpp::direct_workspace_bytes(dp); break;
-    default:
+    if (direct) {
+        need = gpp::direct_workspace_bytes(dp);
+    } else {
         kind = gram_kind(ctx, form);
-        kind.acht = out->acht_n1_loc != nullptr;
+        kind.acht = acht;
         need = bucketed ? gpp::bucketed_workspace_bytes(dp, kind, ctx->bplan) : gpp::factored_workspace_bytes(dp, kind);
-        break;
     }
     if (need == 0) return fail(GPP_ERR_UNSUPPORTED, "form %d is not available for this problem", (int)form);
     HIP_OR_FAIL(ctx->work.reserve(need), "workspace allocation");
@@ -681,25 +693,66 @@ gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
     HIP_OR_FAIL(ctx->results.ensure(sizeof(cplx) * res.count()), "results allocation");
     res.base = static_cast<cplx *>(ctx->results.ptr);
     hipError_t e;
-    if (form == GPP_FORM_DIRECT || form == GPP_FORM_DIRECT_STRICT)
+    if (direct)
         e = gpp::run_direct(dp, form == GPP_FORM_DIRECT_STRICT, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     else if (bucketed)
         e = gpp::run_factored_bucketed(dp, kind, ctx->bplan, ctx->work.ptr, res, ctx->stream, &ctx->timer);
     else
         e = gpp::run_factored(dp, kind, ctx->work.ptr, res, ctx->stream, &ctx->timer);
-    if (e != hipSuccess) return fail_hip(e, "sigma launch");
+    if (e != hipSuccess) {
+        ctx->have_last_form = false;
+        return fail_hip(e, "sigma launch");
+    }
     ctx->last_form = form;
     ctx->have_last_form = true;
-    std::vector<cplx> host(res.count());
-    HIP_OR_FAIL(hipMemcpyAsync(host.data(), res.base, sizeof(cplx) * res.count(), hipMemcpyDeviceToHost, ctx->stream),
+    ctx->last_acht = direct || acht;
+    ctx->timed = true;
+    return GPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gpp_status gpp_sigma_async(gpp_ctx *ctx, gpp_form form, int with_acht_n1_loc)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    return sigma_enqueue(ctx, form, with_acht_n1_loc != 0);
+}
+
+gpp_status gpp_ctx_wait(gpp_ctx *ctx, const gpp_outputs *out)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->have_last_form) return fail(GPP_ERR_NO_INPUTS, "no evaluation enqueued on the current inputs");
+    if (out && out->acht_n1_loc && !ctx->last_acht)
+        return fail(GPP_ERR_UNSUPPORTED, "the last evaluation did not compute acht_n1_loc (request it when enqueuing)");
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    const size_t nf = ctx->dp.nfreq, nb = ctx->dp.nb, count = 2 * nf + nb + 1;
+    if (!out) {
+        HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "sigma synchronize");
+        return GPP_OK;
+    }
+    std::vector<cplx> host(count);
+    HIP_OR_FAIL(hipMemcpyAsync(host.data(), ctx->results.ptr, sizeof(cplx) * count, hipMemcpyDeviceToHost,
+                               ctx->stream),
                 "D2H results");
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "sigma synchronize");
-    ctx->timed = true;
-    memcpy(out->achtemp, host.data(), sizeof(cplx) * dp.nfreq);
-    memcpy(out->asxtemp, host.data() + dp.nfreq, sizeof(cplx) * dp.nfreq);
-    if (out->acht_n1_loc) memcpy(out->acht_n1_loc, host.data() + 2 * dp.nfreq, sizeof(cplx) * dp.nb);
-    if (out->achstemp) memcpy(out->achstemp, host.data() + 2 * dp.nfreq + dp.nb, sizeof(cplx));
+    if (out->achtemp) memcpy(out->achtemp, host.data(), sizeof(cplx) * nf);
+    if (out->asxtemp) memcpy(out->asxtemp, host.data() + nf, sizeof(cplx) * nf);
+    if (out->acht_n1_loc) memcpy(out->acht_n1_loc, host.data() + 2 * nf, sizeof(cplx) * nb);
+    if (out->achstemp) memcpy(out->achstemp, host.data() + 2 * nf + nb, sizeof(cplx));
     return GPP_OK;
+}
+
+gpp_status gpp_sigma(gpp_ctx *ctx, gpp_form form, const gpp_outputs *out)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out || !out->achtemp || !out->asxtemp)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "outputs achtemp and asxtemp are required");
+    gpp_status st = sigma_enqueue(ctx, form, out->acht_n1_loc != nullptr);
+    if (st) return st;
+    return gpp_ctx_wait(ctx, out);
 }
 
 gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
@@ -791,7 +844,7 @@ gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form)
             HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "autotune synchronize");
             if (rep == 0) continue;  // warm-up
             float ms = 0.f;
-            const gpp::StageTimer &t = ctx->timer;
+            const gpp::StageTimer::Slot &t = ctx->timer.last();
             for (int i = 0; i < t.n; ++i) {
                 HIP_OR_FAIL(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]), "hipEventElapsedTime");
                 if (strncmp(t.name[i], "factored_gram", 13) == 0) best_g = std::min(best_g, (double)ms);
@@ -828,7 +881,7 @@ gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *sta
                                int *nstages)
 {
     if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    const gpp::StageTimer &t = ctx->timer;
+    const gpp::StageTimer::Slot &t = ctx->timer.last();
     if (!ctx->timed || t.n == 0) {
         if (total_ms) *total_ms = 0.0;
         if (nstages) *nstages = 0;
@@ -846,10 +899,47 @@ gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *sta
     return GPP_OK;
 }
 
+gpp_status gpp_ctx_timing_summary(const gpp_ctx *ctx, int last_n, int *nevals, double *avg_total_ms,
+                                  double *avg_stage_ms, int max_stages, int *nstages)
+{
+    if (!ctx || last_n < 1) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL or last_n < 1");
+    const gpp::StageTimer &tm = ctx->timer;
+    const gpp::StageTimer::Slot &L = tm.last();
+    if (nevals) *nevals = 0;
+    if (avg_total_ms) *avg_total_ms = 0.0;
+    if (nstages) *nstages = 0;
+    if (!ctx->timed || L.n == 0) return GPP_OK;
+    HIP_OR_FAIL(hipEventSynchronize(L.ev[L.n]), "hipEventSynchronize");
+    const int want = std::min(last_n, tm.filled);
+    double tot = 0.0, st[gpp::StageTimer::kMax] = {};
+    int m = 0;
+    for (int k = 0; k < want; ++k) {
+        const gpp::StageTimer::Slot &S = tm.back(k);
+        bool same = S.n == L.n;
+        for (int i = 0; same && i < L.n; ++i) same = S.name[i] && strcmp(S.name[i], L.name[i]) == 0;
+        if (!same) break;  // an evaluation of another kind: stop at it
+        float ms = 0.f;
+        HIP_OR_FAIL(hipEventElapsedTime(&ms, S.ev[0], S.ev[S.n]), "hipEventElapsedTime");
+        tot += ms;
+        for (int i = 0; i < S.n; ++i) {
+            HIP_OR_FAIL(hipEventElapsedTime(&ms, S.ev[i], S.ev[i + 1]), "hipEventElapsedTime");
+            st[i] += ms;
+        }
+        ++m;
+    }
+    if (nevals) *nevals = m;
+    if (avg_total_ms) *avg_total_ms = tot / m;
+    for (int i = 0; i < L.n && avg_stage_ms && i < max_stages; ++i) avg_stage_ms[i] = st[i] / m;
+    if (nstages) *nstages = L.n;
+    return GPP_OK;
+}
+
 const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i)
 {
-    if (!ctx || i < 0 || i >= ctx->timer.n || !ctx->timer.name[i]) return "";
-    return ctx->timer.name[i];
+    if (!ctx) return "";
+    const gpp::StageTimer::Slot &t = ctx->timer.last();
+    if (i < 0 || i >= t.n || !t.name[i]) return "";
+    return t.name[i];
 }
 
 }  // extern "C"

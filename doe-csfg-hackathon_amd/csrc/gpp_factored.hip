@@ -88,8 +88,9 @@ __global__ void k_gather_aqsm(const cplx *__restrict__ aqsm, const int32_t *__re
         gather_elem(aqsm, idx, ncouls, ngpown, t, bg);
 }
 
-// Grid (ceil(ngpown/TJ), ceil(ncouls/TI)): G' tiles vary fastest, so blocks
-// sharing one aqsn tile run together.
+// 1-D grid over the ceil(ngpown/TJ) x ceil(ncouls/TI) tiles, G' tiles
+// fastest in tile order and tiles spread over the XCDs by xcd_tile, so the
+// blocks sharing one aqsn tile run on one XCD.
 template <int WJN, int WIN>
 __global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restrict__ bg,
                                                                     const cplx *__restrict__ aqsn, int ngpown,
@@ -99,7 +100,10 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restr
     typedef GramCfg<WJN, WIN> C;
     __shared__ cplx sB[2][kKS][C::TJ];
     __shared__ cplx sA[2][kKS][C::TI];
-    gram_f64_tile<WJN, WIN>(bg, aqsn, ngpown, ncouls, k_begin, k_end, blockIdx.x * C::TJ, blockIdx.y * C::TI, sB, sA,
+    const int nj = (ngpown + C::TJ - 1) / C::TJ;
+    const int ni = (ncouls + C::TI - 1) / C::TI;
+    const int t = xcd_tile(blockIdx.x, nj * ni);
+    gram_f64_tile<WJN, WIN>(bg, aqsn, ngpown, ncouls, k_begin, k_end, (t % nj) * C::TJ, (t / nj) * C::TI, sB, sA,
                             mout);
 }
 
@@ -121,8 +125,9 @@ __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach,
     final_block(col_ach, col_asx, part_achs, ncols, nfreq, nb, nv, out, scratch, blockIdx.x);
 }
 
-// Grid (column tiles, j tiles, G-vector splits): the column tiles that share
-// one R tile run together.  part: [jtiles * nks][nb].
+// 1-D grid over (column tiles, j tiles, G-vector splits), column tiles
+// fastest in tile order and spread over the XCDs by xcd_tile, so the blocks
+// that share one R tile run on one XCD.  part: [jtiles * nks][nb].
 template <int WJN, int WIN, bool MASK>
 __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restrict__ p2,
                                                                     const uint8_t *__restrict__ br1,
@@ -131,17 +136,21 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restr
                                                                     const int32_t *__restrict__ idx,
                                                                     const double *__restrict__ vcoul, int ngpown,
                                                                     int ncouls, int nb, int n1_begin, int n1_end,
-                                                                    int chunk, int nks, cplx *__restrict__ part)
+                                                                    int chunk, int nks, int jtiles,
+                                                                    cplx *__restrict__ part)
 {
     typedef AchtCfg<WJN, WIN> C;
     __shared__ cplx sR[2][C::TJ][kAK + 1];
     __shared__ cplx sA[2][C::TN][kAK + 1];
     __shared__ cplx red[WJN][C::TN];
-    const int k_begin = blockIdx.z * chunk;
+    const int nn = (n1_end - n1_begin + C::TN - 1) / C::TN;
+    const int t = xcd_tile(blockIdx.x, nn * jtiles * nks);
+    const int ct = t % nn, jt = (t / nn) % jtiles, ks = t / (nn * jtiles);
+    const int k_begin = ks * chunk;
     const int k_end = k_begin + chunk < ncouls ? k_begin + chunk : ncouls;
     acht_f64_tile<WJN, WIN, MASK>(p2, br1, aqsn, bg, idx, vcoul, ngpown, ncouls, n1_end, k_begin, k_end,
-                                  blockIdx.y * C::TJ, n1_begin + blockIdx.x * C::TN, sR, sA, red,
-                                  part + ((size_t)blockIdx.y * nks + blockIdx.z) * nb);
+                                  jt * C::TJ, n1_begin + ct * C::TN, sR, sA, red,
+                                  part + ((size_t)jt * nks + ks) * nb);
 }
 
 // acht_n1_loc[n1] = sum over the (j tile, split) partials in order.
@@ -158,9 +167,9 @@ template <int WJN, int WIN>
 void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t s)
 {
     typedef GramCfg<WJN, WIN> C;
-    dim3 grid((p.ngpown + C::TJ - 1) / C::TJ, (p.ncouls + C::TI - 1) / C::TI);
-    hipLaunchKernelGGL((k_gram_f64<WJN, WIN>), grid, dim3(C::NT), 0, s, bg, p.aqsn, p.ngpown, p.ncouls, p.nv, p.nb,
-                       mu);
+    const unsigned tiles = (unsigned)((p.ngpown + C::TJ - 1) / C::TJ) * (unsigned)((p.ncouls + C::TI - 1) / C::TI);
+    hipLaunchKernelGGL((k_gram_f64<WJN, WIN>), dim3(tiles), dim3(C::NT), 0, s, bg, p.aqsn, p.ngpown, p.ncouls, p.nv,
+                       p.nb, mu);
 }
 
 template <int FC>
@@ -259,15 +268,16 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
         const AchtSplit &as = L.as;
         cplx *part = reinterpret_cast<cplx *>(w + L.acht_part);
         if (p.nb > p.nv) {
-            dim3 grid((p.nb - p.nv + AchtU::TN - 1) / AchtU::TN, as.jtiles, as.nks);
-            hipLaunchKernelGGL((k_acht_f64<2, 4, false>), grid, dim3(AchtU::NT), 0, s, ra.p2, nullptr, p.aqsn, bg,
-                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, part);
+            const unsigned n = (unsigned)((p.nb - p.nv + AchtU::TN - 1) / AchtU::TN) * as.jtiles * as.nks;
+            hipLaunchKernelGGL((k_acht_f64<2, 4, false>), dim3(n), dim3(AchtU::NT), 0, s, ra.p2, nullptr, p.aqsn, bg,
+                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, as.jtiles,
+                               part);
             GPP_TRY(hipGetLastError());
         }
         if (p.nv > 0) {
-            dim3 grid((p.nv + AchtO::TN - 1) / AchtO::TN, as.jtiles, as.nks);
-            hipLaunchKernelGGL((k_acht_f64<2, 1, true>), grid, dim3(AchtO::NT), 0, s, ra.p2, ra.br1, p.aqsn, bg,
-                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, 0, p.nv, as.chunk, as.nks, part);
+            const unsigned n = (unsigned)((p.nv + AchtO::TN - 1) / AchtO::TN) * as.jtiles * as.nks;
+            hipLaunchKernelGGL((k_acht_f64<2, 1, true>), dim3(n), dim3(AchtO::NT), 0, s, ra.p2, ra.br1, p.aqsn, bg,
+                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, 0, p.nv, as.chunk, as.nks, as.jtiles, part);
             GPP_TRY(hipGetLastError());
         }
         hipLaunchKernelGGL(k_acht_final, dim3((p.nb + 255) / 256), dim3(256), 0, s, part, as.jtiles * as.nks, p.nb,

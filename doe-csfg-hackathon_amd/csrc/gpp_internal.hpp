@@ -48,17 +48,27 @@ struct Workspace {
     void release();
 };
 
-// Records HIP events between the stages of one evaluation.
+// Records HIP events between the stages of an evaluation, for the last
+// kRing evaluations (so that evaluations enqueued back to back can be timed
+// after one synchronisation).
 struct StageTimer {
     static constexpr int kMax = 12;
-    hipEvent_t ev[kMax + 1] = {};
-    const char *name[kMax] = {};
-    int n = 0;        // stages recorded in the current evaluation
+    static constexpr int kRing = 32;
+    struct Slot {
+        hipEvent_t ev[kMax + 1] = {};
+        const char *name[kMax] = {};
+        int n = 0;  // stages recorded
+    };
+    Slot slot[kRing];
+    int cur = 0;     // slot of the current (last) evaluation
+    int filled = 0;  // evaluations recorded, at most kRing
     bool ready = false;
     hipError_t init();
     void destroy();
-    void begin(hipStream_t s);                 // records ev[0]
+    void begin(hipStream_t s);                 // starts a slot, records ev[0]
     void mark(hipStream_t s, const char *nm);  // closes one stage
+    const Slot &last() const { return slot[cur]; }
+    const Slot &back(int k) const { return slot[(cur - k + kRing) % kRing]; }  // k evaluations before the last
 };
 
 // Bytes of device workspace the direct form needs for this problem.

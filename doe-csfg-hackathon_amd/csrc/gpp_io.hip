@@ -107,7 +107,8 @@ gpp_status stream_payload(void *file, const char *path, const Header &h, void *c
     FILE *f = static_cast<FILE *>(file);
     const Layout L = layout_of(h.prob, h.flags);
     if (seek_to(f, kHeaderBytes) != 0) return api_error(GPP_ERR_IO, "%s: seek failed", path);
-    uint64_t hash = kHashInit, pos = 0;
+    Hash hash;
+    uint64_t pos = 0;
     gpp_status st;
     for (int chunk = 0; pos < L.payload; ++chunk) {
         const size_t n = (size_t)std::min<uint64_t>(cap, L.payload - pos);
@@ -117,7 +118,7 @@ gpp_status stream_payload(void *file, const char *path, const Header &h, void *c
         if (fread(buf, 1, n, f) != n)
             return api_error(GPP_ERR_IO, "%s: truncated (payload ends before byte %llu of %llu)", path,
                              (unsigned long long)(pos + n), (unsigned long long)L.payload);
-        hash = hash_words(hash, buf, n);
+        hash.update(buf, n);
         for (int s = 0; s < kSections; ++s) {
             if (!L.bytes[s]) continue;
             const uint64_t lo = std::max(pos, L.off[s]), hi = std::min<uint64_t>(pos + n, L.off[s] + L.bytes[s]);
@@ -126,9 +127,9 @@ gpp_status stream_payload(void *file, const char *path, const Header &h, void *c
         if ((st = sink.after_chunk(b))) return st;
         pos += n;
     }
-    if (hash != h.checksum)
+    if (hash.value() != h.checksum)
         return api_error(GPP_ERR_IO, "%s: checksum mismatch (stored %016llx, computed %016llx)", path,
-                         (unsigned long long)h.checksum, (unsigned long long)hash);
+                         (unsigned long long)h.checksum, (unsigned long long)hash.value());
     return GPP_OK;
 }
 
@@ -207,10 +208,11 @@ gpp_status gpp_file_write(const char *path, const gpp_problem *prob, const int32
     // whole 64-bit words (padding included).
     const size_t cap = (size_t)16 << 20;
     std::vector<unsigned char> stage(cap);
-    uint64_t hash = kHashInit, pos = 0;
+    Hash hash;
+    uint64_t pos = 0;
     size_t fill = 0;
     auto flush = [&]() -> bool {
-        hash = hash_words(hash, stage.data(), fill);
+        hash.update(stage.data(), fill);
         const bool ok = fwrite(stage.data(), 1, fill, f) == fill;
         pos += fill;
         fill = 0;
@@ -242,7 +244,7 @@ gpp_status gpp_file_write(const char *path, const gpp_problem *prob, const int32
         fill += n;
     }
     if (fill && !flush()) return gpp::api_error(GPP_ERR_IO, "%s: write failed", path);
-    h.checksum = hash;
+    h.checksum = hash.value();
     encode_header(h, hb);
     if (fseeko(f, 0, SEEK_SET) != 0 || fwrite(hb, 1, kHeaderBytes, f) != kHeaderBytes)
         return gpp::api_error(GPP_ERR_IO, "%s: header rewrite failed", path);

@@ -10,9 +10,10 @@ the FETCH_SIZE / WRITE_SIZE counters.
     rocprofv3 --pmc FETCH_SIZE -d OUT -o run --output-format csv -- \\
         python3 doe-csfg-hackathon_amd/gpp_profile.py --copy-bytes 1073741824
 
-Workloads: c1 = number_bands 512, nvband 2, ncouls 32768, ngpown 1638,
-nfreq 3 (the benchmark shape); c3 = 1536, 8, 65536, 3276, nfreq 16 with the
-frequency grid 4 + 0.25 (iw - 7.5).
+Workloads (gpp_workloads.py): c1 (the benchmark shape), c3 (the large
+single instance), c4 (64 uneven instances through the batched persistent
+kernels, replayed from a hipGraph).  --acht also requests acht_n1_loc from the
+single-instance forms.
 """
 import argparse
 import ctypes
@@ -23,11 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 import pygpp  # noqa: E402
-
-WORKLOADS = {
-    "c1": (pygpp.Problem(512, 2, 32768, 1638, 3), 4.0, 1.0),
-    "c3": (pygpp.Problem(1536, 8, 65536, 3276, 16), 4.0, 0.25),
-}
+from gpp_workloads import SINGLE, c4_instances  # noqa: E402
 
 
 def copy_calibration(nbytes: int, reps: int) -> None:
@@ -48,7 +45,8 @@ def copy_calibration(nbytes: int, reps: int) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--form", default="direct", choices=sorted(pygpp.FORMS))
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c1", choices=sorted(SINGLE) + ["c4"])
+    ap.add_argument("--acht", action="store_true", help="also request acht_n1_loc")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--copy-bytes", type=int, default=0)
@@ -56,11 +54,23 @@ def main():
     if args.copy_bytes:
         copy_calibration(args.copy_bytes, args.steps)
         return
-    prob, wxc, dw = WORKLOADS[args.workload]
+    if args.workload == "c4":
+        inst = c4_instances()
+        with pygpp.Context(0) as ctx:
+            with pygpp.Batch(ctx, [p for p, _, _, _ in inst], seeds=[s for _, s, _, _ in inst],
+                             wx_center=[w for _, _, w, _ in inst], dw=[d for _, _, _, d in inst]) as batch:
+                batch.autotune(args.form)
+                batch.use_graph(True)
+                for i in range(args.steps):
+                    outs = batch.sigma(args.form)
+                    print(f"step {i}: batch {args.form} c4 {batch.last_timing()['ms']:.3f} ms "
+                          f"achtemp[0] of instance 0 = {outs[0]['achtemp'][0]:.10e}", flush=True)
+        return
+    prob, wxc, dw = SINGLE[args.workload]
     with pygpp.Context(0) as ctx:
         ctx.generate_seeded(prob, args.seed, wxc, dw)
         for i in range(args.steps):
-            out = ctx.sigma(args.form)
+            out = ctx.sigma(args.form, acht_n1_loc=True if args.acht else None)
             t = ctx.last_timing()
             print(f"step {i}: {args.form} {args.workload} total {t['total']:.3f} ms "
                   f"achtemp[0]={out['achtemp'][0]:.10e}", flush=True)

@@ -63,6 +63,9 @@ EXPORTED_SYMBOLS = (
     "gpp_fill_seeded",
     "gpp_generate_seeded",
     "gpp_sigma",
+    "gpp_sigma_async",
+    "gpp_ctx_wait",
+    "gpp_ctx_timing_summary",
     "gpp_set_band_frequencies",
     "gpp_file_write",
     "gpp_file_info",
@@ -159,6 +162,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_fill_seeded.argtypes = [P(GppProblem), ctypes.c_uint64, dbl, dbl] + [vp] * 7
     lib.gpp_generate_seeded.argtypes = [vp, P(GppProblem), ctypes.c_uint64, dbl, dbl]
     lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
+    lib.gpp_sigma_async.argtypes = [vp, i32, i32]
+    lib.gpp_ctx_wait.argtypes = [vp, P(GppOutputs)]
+    lib.gpp_ctx_timing_summary.argtypes = [vp, i32, P(i32), P(dbl), P(dbl), i32, P(i32)]
     lib.gpp_set_band_frequencies.argtypes = [vp, vp]
     lib.gpp_file_write.argtypes = [ctypes.c_char_p, P(GppProblem), vp, vp, i32] + [vp] * 5 + [P(GppOutputs)]
     lib.gpp_file_info.argtypes = [ctypes.c_char_p, P(GppFileHeader)]
@@ -422,6 +428,37 @@ class Context:
         _check(load().gpp_sigma(self._h, FORMS[form], ctypes.byref(o)))
         out["achstemp"] = out["achstemp"][0]
         return out
+
+    def sigma_async(self, form: str = "direct", acht_n1_loc: bool = False) -> None:
+        """Enqueues one evaluation without waiting (see wait)."""
+        _check(load().gpp_sigma_async(self._h, FORMS[form], 1 if acht_n1_loc else 0))
+
+    def wait(self, acht_n1_loc: bool = False) -> Dict[str, np.ndarray]:
+        """Waits for the enqueued evaluations; returns the last one's outputs."""
+        p = self.problem
+        out = {
+            "achtemp": np.zeros(p.nfreq, np.complex128),
+            "asxtemp": np.zeros(p.nfreq, np.complex128),
+            "acht_n1_loc": np.zeros(p.number_bands, np.complex128) if acht_n1_loc else None,
+            "achstemp": np.zeros(1, np.complex128),
+        }
+        o = GppOutputs(*[_ptr(out[k]) for k in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")])
+        _check(load().gpp_ctx_wait(self._h, ctypes.byref(o)))
+        out["achstemp"] = out["achstemp"][0]
+        return out
+
+    def timing_summary(self, last_n: int) -> Dict[str, float]:
+        """Average GPU milliseconds of the last evaluations of the same kind:
+        'evaluations', 'total' and one entry per stage."""
+        lib = load()
+        nev, total, n = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        stages = (ctypes.c_double * 16)()
+        _check(lib.gpp_ctx_timing_summary(self._h, last_n, ctypes.byref(nev), ctypes.byref(total), stages, 16,
+                                          ctypes.byref(n)))
+        res = {"evaluations": nev.value, "total": total.value}
+        for i in range(n.value):
+            res[lib.gpp_ctx_stage_name(self._h, i).decode()] = stages[i]
+        return res
 
     def autotune(self, form: str = "factored-f64") -> Dict[str, int]:
         """Times candidate Gram tiles / reduce schedules and keeps the fastest."""

@@ -261,8 +261,11 @@ gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form);
  * complex[ngpown][ncouls] and (flag GPP_FILE_HAS_EXPECTED) achtemp[nfreq],
  * asxtemp[nfreq], acht_n1_loc[number_bands], achstemp[1], each section
  * starting at a multiple of 64 bytes of the payload with zero padding
- * between.  Checksum: FNV-1a over the payload's 64-bit little-endian words,
- * h = 0xcbf29ce484222325, h = (h ^ w) * 0x100000001b3 mod 2^64.
+ * between.  Checksum: FNV-1a over the payload's 64-bit little-endian words
+ * in four interleaved lanes: lane k (k = 0..3) starts at 0xcbf29ce484222325
+ * and takes the words w_i with i % 4 == k, h_k = (h_k ^ w_i) * 0x100000001b3
+ * mod 2^64; then h = 0xcbf29ce484222325 and h = (h ^ h_k) * 0x100000001b3 for
+ * k = 0..3.
  */
 #define GPP_FILE_WX_PER_BAND 1u
 #define GPP_FILE_HAS_EXPECTED 2u
@@ -301,6 +304,20 @@ gpp_status gpp_file_read(const char *path, int32_t *inv_igp_index, double *wx, d
  * before the context accepts the inputs.  expected as for gpp_file_read. */
 gpp_status gpp_upload_file(gpp_ctx *ctx, const char *path, gpp_outputs *expected);
 
+/*
+ * Enqueues one evaluation on the context's stream and returns without
+ * waiting: evaluations enqueued back to back run without host round trips
+ * (as the reference's CUDA driver launches and synchronises once,
+ * gppCudaVer.cpp:381-395).  The results of the last one stay on the device
+ * until gpp_ctx_wait; with_acht_n1_loc asks the factored forms for
+ * acht_n1_loc (the direct forms always compute it).
+ */
+gpp_status gpp_sigma_async(gpp_ctx *ctx, gpp_form form, int with_acht_n1_loc);
+
+/* Waits for the evaluations enqueued on the context and copies the last one's
+ * outputs into out (may be NULL: only wait; NULL members are skipped). */
+gpp_status gpp_ctx_wait(gpp_ctx *ctx, const gpp_outputs *out);
+
 /* Sets / reads a context option (gpp_option). */
 gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value);
 gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value);
@@ -313,6 +330,15 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
  */
 gpp_status gpp_ctx_last_timing(const gpp_ctx *ctx, double *total_ms, double *stage_ms,
                                int max_stages, int *nstages);
+
+/*
+ * Average GPU time, in milliseconds, of up to the last last_n evaluations
+ * (at most 32 are kept) that ran the same stages as the last one: *nevals
+ * evaluations averaged, total and per stage as gpp_ctx_last_timing.  Any
+ * output pointer may be NULL.
+ */
+gpp_status gpp_ctx_timing_summary(const gpp_ctx *ctx, int last_n, int *nevals, double *avg_total_ms,
+                                  double *avg_stage_ms, int max_stages, int *nstages);
 
 /* Name of stage i of the last gpp_sigma ("" when out of range). */
 const char *gpp_ctx_stage_name(const gpp_ctx *ctx, int i);

@@ -29,9 +29,16 @@ def parse_independently(path):
     assert version == 1 and zero == 0 and raw[64:128] == bytes(64)
     body = raw[128:]
     assert len(body) == payload and payload % 64 == 0
+    words = np.frombuffer(body, "<u8").tolist()
+    lanes = []
+    for k in range(4):
+        h = 0xcbf29ce484222325
+        for w in words[k::4]:
+            h = ((h ^ w) * 0x100000001b3) & MASK64
+        lanes.append(h)
     h = 0xcbf29ce484222325
-    for w in np.frombuffer(body, "<u8").tolist():
-        h = ((h ^ w) * 0x100000001b3) & MASK64
+    for x in lanes:
+        h = ((h ^ x) * 0x100000001b3) & MASK64
     assert h == checksum
     sections = [("inv_igp_index", "<i4", ng), ("wx_array", "<f8", nf * (nb if flags & 1 else 1)),
                 ("vcoul", "<f8", nc), ("aqsmtemp", "<c16", nb * nc), ("aqsntemp", "<c16", nb * nc),
