@@ -1136,7 +1136,7 @@ gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
         if (!outs[i].achtemp || !outs[i].asxtemp)
             return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: achtemp and asxtemp are required", i);
         if (outs[i].acht_n1_loc)
-            return fail(GPP_ERR_UNSUPPORTED, "instance %d: acht_n1_loc comes from the direct forms only", i);
+            return fail(GPP_ERR_UNSUPPORTED, "instance %d: batches do not compute acht_n1_loc; pass NULL", i);
     }
     gpp_status st = set_device(b->ctx);
     if (st) return st;

@@ -12,9 +12,11 @@
 //                      gppComplex.cpp:66-101) from k_achstemp.
 // When acht_n1_loc is requested (GramKind::acht), stage 3 also writes the
 // pole factor p2 of frequency 2 (and the occupied branch mask), and
-//   5. k_acht_f64      second FP64 MFMA product Y = R * aqsn^T, split over
-//                      G-vectors, reduced per band in its epilogue
-//   6. k_acht_final    fixed-order sum of the per-tile band partials.
+//   5. k_acht_f64      second FP64 MFMA product Y = P2 * aqsn^T for the
+//                      unoccupied bands, split over G-vectors, reduced per
+//                      band in its epilogue; k_acht_occ streams the masked
+//                      P2 against the few occupied aqsn rows
+//   6. k_acht_final    fixed-order sum of the per-band partials.
 #include <string.h>
 
 #include <algorithm>
@@ -31,7 +33,6 @@ __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // acht product tiles: rows j in 128s for both band ranges; unoccupied bands
 // in columns of 128, occupied bands (few) in columns of 32.
 typedef AchtCfg<2, 4> AchtU;
-typedef AchtCfg<2, 1> AchtO;
 
 struct AchtSplit {
     int jtiles, nks, chunk;
@@ -53,7 +54,7 @@ AchtSplit acht_split(const DevProblem &p)
 }
 
 struct FactLayout {
-    size_t bg, m, col_ach, col_asx, part_achs, p2, br1, acht_part, total;
+    size_t bg, m, col_ach, col_asx, part_achs, p2, br1, acht_part, acht_occ, total;
     int nsplit;
     AchtSplit as;
 };
@@ -68,13 +69,14 @@ FactLayout fact_layout(const DevProblem &p, const GramKind &k)
     L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
-    L.p2 = L.br1 = L.acht_part = off;
+    L.p2 = L.br1 = L.acht_part = L.acht_occ = off;
     L.as = AchtSplit{0, 0, 0};
     if (k.acht) {
         L.as = acht_split(p);
         L.p2 = off;        off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
         L.br1 = off;       off += p.nv > 0 ? align256((size_t)p.ngpown * p.ncouls) : 0;
         L.acht_part = off; off += align256(sizeof(cplx) * (size_t)L.as.jtiles * L.as.nks * p.nb);
+        L.acht_occ = off;  off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * (p.nv > 0 ? p.nv : 1));
     }
     L.total = off;
     return L;
@@ -128,9 +130,8 @@ __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach,
 // 1-D grid over (column tiles, j tiles, G-vector splits), column tiles
 // fastest in tile order and spread over the XCDs by xcd_tile, so the blocks
 // that share one R tile run on one XCD.  part: [jtiles * nks][nb].
-template <int WJN, int WIN, bool MASK>
+template <int WJN, int WIN>
 __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restrict__ p2,
-                                                                    const uint8_t *__restrict__ br1,
                                                                     const cplx *__restrict__ aqsn,
                                                                     const cplx *__restrict__ bg,
                                                                     const int32_t *__restrict__ idx,
@@ -140,27 +141,58 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restr
                                                                     cplx *__restrict__ part)
 {
     typedef AchtCfg<WJN, WIN> C;
-    __shared__ cplx sR[2][C::TJ][kAK + 1];
-    __shared__ cplx sA[2][C::TN][kAK + 1];
+    __shared__ cplx sR[2][C::TJ][kAK];
+    __shared__ cplx sA[2][C::TN][kAK];
     __shared__ cplx red[WJN][C::TN];
     const int nn = (n1_end - n1_begin + C::TN - 1) / C::TN;
     const int t = xcd_tile(blockIdx.x, nn * jtiles * nks);
     const int ct = t % nn, jt = (t / nn) % jtiles, ks = t / (nn * jtiles);
     const int k_begin = ks * chunk;
     const int k_end = k_begin + chunk < ncouls ? k_begin + chunk : ncouls;
-    acht_f64_tile<WJN, WIN, MASK>(p2, br1, aqsn, bg, idx, vcoul, ngpown, ncouls, n1_end, k_begin, k_end,
+    acht_f64_tile<WJN, WIN>(p2, aqsn, bg, idx, vcoul, ngpown, ncouls, n1_end, k_begin, k_end,
                                   jt * C::TJ, n1_begin + ct * C::TN, sR, sA, red,
                                   part + ((size_t)jt * nks + ks) * nb);
 }
 
-// acht_n1_loc[n1] = sum over the (j tile, split) partials in order.
-__global__ void k_acht_final(const cplx *__restrict__ part, int nparts, int nb, cplx *__restrict__ out)
+// Grid (ngpown, nsplit): the occupied bands' acht_n1_loc partials.
+__global__ __launch_bounds__(kRedThreads) void k_acht_occ(const cplx *__restrict__ p2,
+                                                          const uint8_t *__restrict__ br1,
+                                                          const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
+                                                          const int32_t *__restrict__ idx,
+                                                          const double *__restrict__ vcoul, int ngpown, int ncouls,
+                                                          int nv, int chunk, int nsplit, cplx *__restrict__ part_occ)
 {
-    const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n1 >= nb) return;
+    __shared__ cplx scratch[kRedThreads / kWave];
+    acht_occ_item(p2, br1, aqsn, bg, idx, vcoul, ngpown, ncouls, nv, chunk, blockIdx.x, blockIdx.y,
+                  part_occ + ((size_t)blockIdx.x * nsplit + blockIdx.y) * nv, scratch);
+}
+
+// acht_n1_loc[n1] as the fixed-order sum of its partials: the (j tile, split)
+// partials of the product for n1 >= nv, the (column, split) partials of
+// k_acht_occ for n1 < nv.  Block = 64 bands x 4 quarters of the partials,
+// quarters combined in order.
+__global__ __launch_bounds__(256) void k_acht_final(const cplx *__restrict__ part, int nparts,
+                                                    const cplx *__restrict__ part_occ, int nparts_occ, int nb,
+                                                    int nv, cplx *__restrict__ out)
+{
+    __shared__ cplx quarter[4][64];
+    const int n1 = blockIdx.x * 64 + threadIdx.x % 64;
+    const int qt = threadIdx.x / 64;
     cplx s = czero();
-    for (int q = 0; q < nparts; ++q) s = cadd(s, part[(size_t)q * nb + n1]);
-    out[n1] = s;
+    if (n1 < nb) {
+        const bool occ = n1 < nv;
+        const int np = occ ? nparts_occ : nparts;
+        const int q0 = (int)((long long)np * qt / 4), q1 = (int)((long long)np * (qt + 1) / 4);
+        for (int q = q0; q < q1; ++q)
+            s = cadd(s, occ ? part_occ[(size_t)q * nv + n1] : part[(size_t)q * nb + n1]);
+    }
+    quarter[qt][threadIdx.x % 64] = s;
+    __syncthreads();
+    if (qt == 0 && n1 < nb) {
+        cplx t = quarter[0][threadIdx.x];
+        for (int k = 1; k < 4; ++k) t = cadd(t, quarter[k][threadIdx.x]);
+        out[n1] = t;
+    }
 }
 
 template <int WJN, int WIN>
@@ -267,21 +299,20 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     if (kind.acht) {
         const AchtSplit &as = L.as;
         cplx *part = reinterpret_cast<cplx *>(w + L.acht_part);
+        cplx *part_occ = reinterpret_cast<cplx *>(w + L.acht_occ);
         if (p.nb > p.nv) {
             const unsigned n = (unsigned)((p.nb - p.nv + AchtU::TN - 1) / AchtU::TN) * as.jtiles * as.nks;
-            hipLaunchKernelGGL((k_acht_f64<2, 4, false>), dim3(n), dim3(AchtU::NT), 0, s, ra.p2, nullptr, p.aqsn, bg,
-                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, as.jtiles,
-                               part);
+            hipLaunchKernelGGL((k_acht_f64<2, 4>), dim3(n), dim3(AchtU::NT), 0, s, ra.p2, p.aqsn, bg, p.idx,
+                               p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, as.jtiles, part);
             GPP_TRY(hipGetLastError());
         }
         if (p.nv > 0) {
-            const unsigned n = (unsigned)((p.nv + AchtO::TN - 1) / AchtO::TN) * as.jtiles * as.nks;
-            hipLaunchKernelGGL((k_acht_f64<2, 1, true>), dim3(n), dim3(AchtO::NT), 0, s, ra.p2, ra.br1, p.aqsn, bg,
-                               p.idx, p.vcoul, p.ngpown, p.ncouls, p.nb, 0, p.nv, as.chunk, as.nks, as.jtiles, part);
+            hipLaunchKernelGGL(k_acht_occ, dim3(p.ngpown, L.nsplit), dim3(kRedThreads), 0, s, ra.p2, ra.br1, p.aqsn,
+                               bg, p.idx, p.vcoul, p.ngpown, p.ncouls, p.nv, kind.red_chunk, L.nsplit, part_occ);
             GPP_TRY(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_acht_final, dim3((p.nb + 255) / 256), dim3(256), 0, s, part, as.jtiles * as.nks, p.nb,
-                           res.acht_n1_loc());
+        hipLaunchKernelGGL(k_acht_final, dim3((p.nb + 63) / 64), dim3(256), 0, s, part, as.jtiles * as.nks, part_occ,
+                           p.ngpown * L.nsplit, p.nb, p.nv, res.acht_n1_loc());
         GPP_TRY(hipGetLastError());
         if (timer) timer->mark(s, "factored_acht");
     }

@@ -284,27 +284,30 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 // acht_n1_loc (gppComplex.cpp:381, sch_array[2] of each band and column):
 //   acht[n1] = sum_j 0.5 vcoul[igp(j)] conj(aqsm[n1,igp(j)]) sum_ig aqsn[n1,ig] R[j,ig]
 // with R = p2 (P = delw*I_eps at that frequency) for the unoccupied bands and
-// R = p2 masked by br1 (sch of ssxt_scht_solver, :37) for the occupied ones.
-// Y = R * aqsn^T is a second Gram-type product (rows j, columns n1, K = G);
+// R = p2 masked by br1 (sch of ssxt_scht_solver, :37) for the occupied ones
+// (acht_occ_item below: few bands, a streaming pass).  For the unoccupied
+// bands Y = R * aqsn^T is a second Gram-type product (rows j, columns n1, K = G);
 // one block computes a TJ x TN tile of Y over G-vectors [k_begin, k_end) on
 // v_mfma_f64_16x16x4f64 and reduces it at once to the column sums
 // sum_j w_j conj(bg[n1][j]) Y[j][n1], written to part[n1] (one entry per
-// column of the tile).  Operand tiles are staged K-fastest with one complex
-// of padding per row, so both the coalesced global loads and the MFMA reads
-// hit distinct LDS banks.
+// column of the tile).  Operand tiles are staged K-fastest in 128-byte rows
+// of 8 complex, element (row, k) at slot k ^ (row & 7): the stores (8 lanes
+// per row) and the MFMA reads (16 rows per 16 lanes) are then free of LDS
+// bank conflicts on gfx950 (measured with SQ_LDS_BANK_CONFLICT; a padded
+// pitch of 9 complex is not).
 constexpr int kAK = 8;  // G-vectors per LDS stage of the acht product
 
 template <int WJN, int WIN> struct AchtCfg {
     static constexpr int TJ = kWJ * WJN, TN = kWI * WIN, NT = 64 * WJN * WIN;
 };
 
-template <int WJN, int WIN, bool MASK>
-__device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const uint8_t *__restrict__ br1,
+template <int WJN, int WIN>
+__device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
                                               const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
                                               const int32_t *__restrict__ idx, const double *__restrict__ vcoul,
                                               int ngpown, int ncouls, int n1_end, int k_begin, int k_end, int j0,
-                                              int c0, cplx (*sR)[AchtCfg<WJN, WIN>::TJ][kAK + 1],
-                                              cplx (*sA)[AchtCfg<WJN, WIN>::TN][kAK + 1],
+                                              int c0, cplx (*sR)[AchtCfg<WJN, WIN>::TJ][kAK],
+                                              cplx (*sA)[AchtCfg<WJN, WIN>::TN][kAK],
                                               cplx (*red)[AchtCfg<WJN, WIN>::TN], cplx *__restrict__ part)
 {
     typedef AchtCfg<WJN, WIN> C;
@@ -332,10 +335,7 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const
         for (int q = 0; q < PR; ++q) {
             const int e = threadIdx.x + q * NT;
             const int r = e / kAK, k = k0 + e % kAK, j = j0 + r;
-            const size_t o = (size_t)j * ncouls + k;
-            cplx v = czero();
-            if (k < k_end && j < ngpown && (!MASK || br1[o])) v = p2[o];
-            stR[q] = v;
+            stR[q] = (k < k_end && j < ngpown) ? p2[(size_t)j * ncouls + k] : czero();
         }
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
@@ -348,12 +348,14 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const
 #pragma unroll
         for (int q = 0; q < PR; ++q) {
             const int e = threadIdx.x + q * NT;
-            sR[buf][e / kAK][e % kAK] = stR[q];
+            const int r = e / kAK;
+            sR[buf][r][(e % kAK) ^ (r & 7)] = stR[q];
         }
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
             const int e = threadIdx.x + q * NT;
-            sA[buf][e / kAK][e % kAK] = stA[q];
+            const int r = e / kAK;
+            sA[buf][r][(e % kAK) ^ (r & 7)] = stA[q];
         }
     };
 
@@ -366,7 +368,7 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const
         if (st + 1 < nst) load_stage(k_begin + (st + 1) * kAK);
 #pragma unroll
         for (int kk = 0; kk < kAK; kk += 4) {
-            const int kr = kk + lane / 16;
+            const int kr = (kk + lane / 16) ^ (lane & 7);  // every row read below is = lane (mod 8)
             double rre[4], rim[4], rneg[4], are[2], aim[2];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
@@ -429,6 +431,43 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2, const
 #pragma unroll
         for (int w = 0; w < WJN; ++w) s = cadd(s, red[w][col]);
         if (c0 + col < n1_end) part[c0 + col] = s;
+    }
+}
+
+// acht_n1_loc of the occupied bands n1 < nv for column j over G-vectors
+// [split*chunk, +chunk): sum_ig aqsn[n1,ig] p2[j,ig] where the occupied solver
+// took branch 1 (sch = P, else 0), bands in groups of 8 held in registers,
+// block-wide fixed-order sums, weighted by 0.5 vcoul[igp(j)]
+// conj(aqsm[n1,igp(j)]) and written to part[n1] (nv entries).
+__device__ __forceinline__ void acht_occ_item(const cplx *__restrict__ p2, const uint8_t *__restrict__ br1,
+                                              const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
+                                              const int32_t *__restrict__ idx, const double *__restrict__ vcoul,
+                                              int ngpown, int ncouls, int nv, int chunk, int j, int split,
+                                              cplx *__restrict__ part, cplx *scratch)
+{
+    constexpr int G = 8;
+    const int ig_begin = split * chunk;
+    const int ig_end = ig_begin + chunk < ncouls ? ig_begin + chunk : ncouls;
+    const size_t row = (size_t)j * ncouls;
+    const double w = 0.5 * vcoul[igp_of(idx, j, ncouls)];
+    for (int n0 = 0; n0 < nv; n0 += G) {
+        cplx acc[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) acc[u] = czero();
+        for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
+            if (!br1[row + ig]) continue;
+            const cplx q = p2[row + ig];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (n0 + u < nv) acc[u] = cfma(aqsn[(size_t)(n0 + u) * ncouls + ig], q, acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (n0 + u >= nv) break;  // uniform
+            const cplx s = block_sum(acc[u], scratch);
+            if (threadIdx.x == 0)
+                part[n0 + u] = cscale(cmul_fast(cconj(bg[(size_t)(n0 + u) * ngpown + j]), s), w);
+        }
     }
 }
 
