@@ -24,7 +24,8 @@
 //
 // The same kernel runs with INT8 digits (v_mfma_i32_16x16x64_i8): digits in
 // [-64, 64] are exact int8, products accumulate exactly in INT32 while
-// 6 * 2 * 4096 * K < 2^31 (K < 43690 bands), so no chunking is needed; MI355X
+// 6 * 2 * 4096 * K < 2^31 (K < 43690 bands), so the levels are folded to FP64
+// once, after the whole band range; MI355X
 // runs INT8 MFMA at twice the FP16 rate and the staged digits are half the bytes.
 //
 // MFMA lane layouts (measured on gfx950), 16 bytes of operand per lane:
@@ -42,7 +43,12 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef signed char v16c __attribute__((ext_vector_type(16)));
 
 constexpr int kOzTile = 64;     // block tile: 64 G' columns x 64 G-vectors
-constexpr int kOzRowBytes = 80; // LDS row pitch: 64 bytes of digits + 16 pad
+// LDS: per digit plane p and tile row (a column j or ig), one 128-byte row
+// holding the stage's real-part digits (16-byte units 0-3) and imaginary-part
+// digits (units 4-7); logical unit u of row r sits in slot u ^ (r & 7), which
+// makes both the contiguous stores and the 16-row MFMA fragment reads free of
+// LDS bank conflicts on gfx950 (a padded 80-byte pitch is not).
+constexpr int kOzRowBytes = 128;
 
 // Wave layouts of the 64 x 64 block tile: NWJ x NWI waves, each owning
 // SJ x SI sub-tiles of 16 x 16 (NWJ*SJ*16 = NWI*SI*16 = 64).
@@ -60,6 +66,7 @@ template <> struct OzDigit<_Float16> {
     typedef v4f acc;
     static constexpr int KS = 32;       // bands per MFMA k-step = per LDS stage (64 bytes)
     static constexpr int CHUNK = 256;   // bands per exact accumulation chunk
+    static constexpr bool ONE_CHUNK = false;
     static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -70,7 +77,7 @@ template <> struct OzDigit<int8_t> {
     typedef v4i frag;
     typedef v4i acc;
     static constexpr int KS = 64;
-    static constexpr int CHUNK = 32768;
+    static constexpr bool ONE_CHUNK = true;  // exact over every admissible band count (K < 43690)
     static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
     {
         return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
@@ -87,36 +94,54 @@ __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Column exponent e with 2^e > max over rows [k_begin, k_end) of |re|, |im| of
 // column c of a row-major complex matrix (row stride ld).  Zero columns get 0.
-__global__ void k_oz_colexp(const cplx *__restrict__ src, int ld, int k_begin, int k_end, int ncols, int ncols_pad,
-                            int *__restrict__ exps)
+// Block = 64 columns x 4 row groups (rows strided by 4), combined in LDS.
+__global__ __launch_bounds__(256) void k_oz_colexp(const cplx *__restrict__ src, int ld, int k_begin, int k_end,
+                                                   int ncols, int ncols_pad, int *__restrict__ exps)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols_pad) return;
-    double m = 0.0;
-    if (c < ncols)
-        for (int k = k_begin; k < k_end; ++k) {
-            const cplx v = src[(size_t)k * ld + c];
-            m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
+    __shared__ double part[4][64];
+    const int cl = threadIdx.x % 64, rg = threadIdx.x / 64;
+    const int c = blockIdx.x * 64 + cl;
+    double m0 = 0.0, m1 = 0.0;
+    if (c < ncols) {
+        int k = k_begin + rg;
+        for (; k + 4 < k_end; k += 8) {
+            const cplx v = src[(size_t)k * ld + c], w = src[(size_t)(k + 4) * ld + c];
+            m0 = fmax(m0, fmax(fabs(v.re), fabs(v.im)));
+            m1 = fmax(m1, fmax(fabs(w.re), fabs(w.im)));
         }
-    int e = 0;
-    if (m > 0.0) (void)frexp(m, &e);  // m = f 2^e, f in [0.5, 1): 2^e > m
-    exps[c] = e;
+        if (k < k_end) {
+            const cplx v = src[(size_t)k * ld + c];
+            m0 = fmax(m0, fmax(fabs(v.re), fabs(v.im)));
+        }
+    }
+    part[rg][cl] = fmax(m0, m1);
+    __syncthreads();
+    if (rg == 0 && c < ncols_pad) {
+        const double m = fmax(fmax(part[0][cl], part[1][cl]), fmax(part[2][cl], part[3][cl]));
+        int e = 0;
+        if (m > 0.0) (void)frexp(m, &e);  // m = f 2^e, f in [0.5, 1): 2^e > m
+        exps[c] = e;
+    }
 }
 
-// Digits of column c, bands [k_begin + 8*kg, +8), into
-// out[((p*2 + part) * ncols_pad + c) * kpad + k] (k contiguous).
+// Digits of column c, bands [k_begin + KP*kq, +KP) (KP = 16 bytes of digits),
+// into the 16-byte piece out + (((p*2 + part) * (kpad/KP) + kq) * ncols_pad + c) * KP:
+// consecutive columns' pieces are contiguous, so both the column-wise reads of
+// src and the writes are coalesced.  Grid (ncols_pad/256, kpad/KP).
 template <int S, typename D>
-__global__ void k_oz_split(const cplx *__restrict__ src, int ld, int k_begin, int K, int ncols, int ncols_pad,
-                           int kpad, const int *__restrict__ exps, D *__restrict__ out)
+__global__ __launch_bounds__(256) void k_oz_split(const cplx *__restrict__ src, int ld, int k_begin, int K,
+                                                  int ncols, int ncols_pad, int kpad,
+                                                  const int *__restrict__ exps, D *__restrict__ out)
 {
+    constexpr int KP = 16 / (int)sizeof(D);
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kg = blockIdx.y;
+    const int kq = blockIdx.y;
     if (c >= ncols_pad) return;
     const int e = exps[c];
-    D dre[S][8], dim[S][8];
+    D dre[S][KP], dim[S][KP];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        const int k = kg * 8 + kk;
+    for (int kk = 0; kk < KP; ++kk) {
+        const int k = kq * KP + kk;
         cplx v = czero();
         if (c < ncols && k < K) v = src[(size_t)(k_begin + k) * ld + c];
         double yr = ldexp(v.re, 6 - e), yi = ldexp(v.im, 6 - e);  // |y| < 64, exact
@@ -129,10 +154,11 @@ __global__ void k_oz_split(const cplx *__restrict__ src, int ld, int k_begin, in
             yi = (yi - di) * 128.0;
         }
     }
+    const size_t npieces = (size_t)(kpad / KP);
 #pragma unroll
     for (int p = 0; p < S; ++p) {
-        __builtin_memcpy(out + ((size_t)(p * 2 + 0) * ncols_pad + c) * kpad + kg * 8, dre[p], 8 * sizeof(D));
-        __builtin_memcpy(out + ((size_t)(p * 2 + 1) * ncols_pad + c) * kpad + kg * 8, dim[p], 8 * sizeof(D));
+        __builtin_memcpy(out + (((size_t)(p * 2 + 0) * npieces + kq) * ncols_pad + c) * KP, dre[p], 16);
+        __builtin_memcpy(out + (((size_t)(p * 2 + 1) * npieces + kq) * ncols_pad + c) * KP, dim[p], 16);
     }
 }
 
@@ -149,27 +175,42 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
     typedef typename T::frag frag;
     typedef typename T::acc acc_t;
     constexpr int KS = T::KS;
-    __shared__ __attribute__((aligned(16))) unsigned char sB[S * 2][kOzTile][kOzRowBytes];
-    __shared__ __attribute__((aligned(16))) unsigned char sA[S * 2][kOzTile][kOzRowBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[S][kOzTile][kOzRowBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sA[S][kOzTile][kOzRowBytes];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wj = (wave / W::NWI) * (16 * SJ), wi = (wave % W::NWI) * (16 * SI);
     const int j0 = blockIdx.x * kOzTile, i0 = blockIdx.y * kOzTile;
-    const int fo = 16 * (lane / 16);  // fragment byte offset within a 64-byte row
+    // Fragment slots: every row this lane reads is = lane (mod 8).
+    const int fo_re = 16 * ((lane / 16) ^ (lane & 7));
+    const int fo_im = 16 * ((4 + lane / 16) ^ (lane & 7));
 
+    // Level accumulators; with chunked exact sums (FP16) also the FP64 running
+    // sums of the chunks (with INT8 the levels are exact over the whole band
+    // range and are folded once, in the epilogue).
     acc_t acc_re[S][SJ][SI], acc_im[S][SJ][SI];
-    double m_re[SJ][SI][4], m_im[SJ][SI][4];
+    constexpr int NM = T::ONE_CHUNK ? 1 : SJ * SI * 4;
+    double m_re[NM], m_im[NM];
 #pragma unroll
     for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-        for (int si = 0; si < SI; ++si) {
+        for (int si = 0; si < SI; ++si)
 #pragma unroll
             for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m_re[sj][si][r] = m_im[sj][si][r] = 0.0;
+    for (int i = 0; i < NM; ++i) m_re[i] = m_im[i] = 0.0;
+    // Horner over the levels (smallest first) of result r of sub-tile (sj, si).
+    auto fold = [&](int sj, int si, int r, double &sr, double &sm) {
+        sr = 0.0;
+        sm = 0.0;
+#pragma unroll
+        for (int t = S - 1; t >= 0; --t) {
+            sr = sr * 0x1p-7 + (double)acc_re[t][sj][si][r];
+            sm = sm * 0x1p-7 + (double)acc_im[t][sj][si][r];
         }
+    };
 
-    // Fold the level accumulators into the FP64 sums (smallest level first).
+    // Fold the level accumulators into the FP64 sums (chunked form only).
     auto flush = [&]() {
 #pragma unroll
         for (int sj = 0; sj < SJ; ++sj)
@@ -177,14 +218,11 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
             for (int si = 0; si < SI; ++si)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    double sr = 0.0, sm = 0.0;
-#pragma unroll
-                    for (int t = S - 1; t >= 0; --t) {
-                        sr = sr * 0x1p-7 + (double)acc_re[t][sj][si][r];
-                        sm = sm * 0x1p-7 + (double)acc_im[t][sj][si][r];
-                    }
-                    m_re[sj][si][r] += sr;
-                    m_im[sj][si][r] += sm;
+                    double sr, sm;
+                    fold(sj, si, r, sr, sm);
+                    const int i = T::ONE_CHUNK ? 0 : (sj * SI + si) * 4 + r;
+                    m_re[i] += sr;
+                    m_im[i] += sm;
                 }
 #pragma unroll
         for (int sj = 0; sj < SJ; ++sj)
@@ -194,68 +232,101 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
                 for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
     };
 
-    const size_t rowb = (size_t)kpad * sizeof(D);  // bytes per digit row in HBM
+    // Digit planes in HBM: 16-byte pieces (KP bands of one column), pieces of
+    // consecutive columns contiguous, planes of kpad/KP piece rows.
+    constexpr int KP = 16 / (int)sizeof(D);
+    const int npieces = kpad / KP;
     const unsigned char *gb = reinterpret_cast<const unsigned char *>(bd);
     const unsigned char *ga = reinterpret_cast<const unsigned char *>(ad);
-    // 16-byte staging units per stage: 2S planes x 64 rows x 4 per side.
-    constexpr int UNITS = S * 2 * kOzTile * 4;
-    for (int k0 = 0; k0 < kpad; k0 += KS) {
-        const size_t kb = (size_t)k0 * sizeof(D);
-        for (int u = threadIdx.x; u < 2 * UNITS; u += NTH) {
+    // 16-byte staging units per stage: S planes x 64 rows x 8 per side.  The
+    // next stage's units are loaded into registers while the current stage
+    // computes, then stored to LDS between two barriers.
+    constexpr int UNITS = S * kOzTile * 8;
+    constexpr int PER_T = 2 * UNITS / NTH;
+    static_assert(PER_T * NTH == 2 * UNITS, "stage must split evenly");
+    v4i stage_reg[PER_T];
+    auto load_stage = [&](int k0) {
+        const int kq0 = k0 / KP;  // a stage is 4 pieces
+#pragma unroll
+        for (int q = 0; q < PER_T; ++q) {
+            const int u = threadIdx.x + q * NTH;
             const int side = u / UNITS;
             const int v = u % UNITS;
-            const int q4 = v % 4, row = (v / 4) % kOzTile, plane = v / (4 * kOzTile);  // plane = p*2 + part
+            const int slot = v % 8, row = (v / 8) % kOzTile, p = v / (8 * kOzTile);
+            const int lu = slot ^ (row & 7);  // logical unit: part = lu / 4, 16-byte piece lu % 4
             const unsigned char *src = side ? ga : gb;
             const int npad = side ? nc_pad : ngp_pad;
             const int col = (side ? i0 : j0) + row;
-            const v4i val = *reinterpret_cast<const v4i *>(src + ((size_t)plane * npad + col) * rowb + kb + q4 * 16);
-            unsigned char *dst = side ? &sA[plane][row][q4 * 16] : &sB[plane][row][q4 * 16];
-            *reinterpret_cast<v4i *>(dst) = val;
+            stage_reg[q] = *reinterpret_cast<const v4i *>(
+                src + (((size_t)(p * 2 + lu / 4) * npieces + kq0 + lu % 4) * npad + col) * 16);
         }
-        __syncthreads();
-        // G-vector side fragments of every digit, kept in registers for the stage.
-        frag fa[S][SJ][SI];
+    };
+    auto store_stage = [&]() {
 #pragma unroll
-        for (int q = 0; q < S; ++q)
-#pragma unroll
-            for (int part = 0; part < 2; ++part)
-#pragma unroll
-                for (int si = 0; si < SI; ++si)
-                    fa[q][part][si] = *reinterpret_cast<const frag *>(&sA[q * 2 + part][wi + 16 * si + lane % 16][fo]);
+        for (int q = 0; q < PER_T; ++q) {
+            const int u = threadIdx.x + q * NTH;
+            const int side = u / UNITS;
+            const int v = u % UNITS;
+            const int slot = v % 8, row = (v / 8) % kOzTile, p = v / (8 * kOzTile);
+            unsigned char *dst = side ? &sA[p][row][slot * 16] : &sB[p][row][slot * 16];
+            *reinterpret_cast<v4i *>(dst) = stage_reg[q];
+        }
+    };
+    load_stage(0);
+    store_stage();
+    __syncthreads();
+    for (int k0 = 0; k0 < kpad; k0 += KS) {
+        if (k0 + KS < kpad) load_stage(k0 + KS);
+        // G' side fragments of digit p are held across its pairs; the G-vector
+        // side fragments of digit q are read from LDS for each pair (fewer
+        // live registers, which leaves room for the next stage's loads).
 #pragma unroll
         for (int p = 0; p < S; ++p) {
             frag br[SJ], bi[SJ], bn[SJ];
 #pragma unroll
             for (int sj = 0; sj < SJ; ++sj) {
-                br[sj] = *reinterpret_cast<const frag *>(&sB[p * 2 + 0][wj + 16 * sj + lane % 16][fo]);
-                bi[sj] = *reinterpret_cast<const frag *>(&sB[p * 2 + 1][wj + 16 * sj + lane % 16][fo]);
+                br[sj] = *reinterpret_cast<const frag *>(&sB[p][wj + 16 * sj + lane % 16][fo_re]);
+                bi[sj] = *reinterpret_cast<const frag *>(&sB[p][wj + 16 * sj + lane % 16][fo_im]);
                 bn[sj] = T::neg(bi[sj]);
             }
 #pragma unroll
             for (int q = 0; q + p < S; ++q) {
                 const int t = p + q;
+                frag fa[2][SI];  // [real, imaginary part][sub-tile] of digit q
+#pragma unroll
+                for (int part = 0; part < 2; ++part)
+#pragma unroll
+                    for (int si = 0; si < SI; ++si)
+                        fa[part][si] = *reinterpret_cast<const frag *>(
+                            &sA[q][wi + 16 * si + lane % 16][part ? fo_im : fo_re]);
                 // conj(b) * a: re = br ar + bi ai, im = br ai - bi ar.  Each group of
                 // four MFMAs writes four different accumulators.
 #pragma unroll
                 for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(br[sj], fa[q][0][si], acc_re[t][sj][si]);
+                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(br[sj], fa[0][si], acc_re[t][sj][si]);
 #pragma unroll
                 for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(br[sj], fa[q][1][si], acc_im[t][sj][si]);
+                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(br[sj], fa[1][si], acc_im[t][sj][si]);
 #pragma unroll
                 for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(bi[sj], fa[q][1][si], acc_re[t][sj][si]);
+                    for (int si = 0; si < SI; ++si) acc_re[t][sj][si] = T::mfma(bi[sj], fa[1][si], acc_re[t][sj][si]);
 #pragma unroll
                 for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(bn[sj], fa[q][0][si], acc_im[t][sj][si]);
+                    for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(bn[sj], fa[0][si], acc_im[t][sj][si]);
             }
         }
         __syncthreads();
-        if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush();
+        if constexpr (!T::ONE_CHUNK) {
+            if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush();
+        }
+        if (k0 + KS < kpad) {
+            store_stage();
+            __syncthreads();
+        }
     }
 
 #pragma unroll
@@ -268,7 +339,14 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
                 const int ig = i0 + wi + 16 * si + lane % 16;
                 if (j < ngpown && ig < ncouls) {
                     const int sc = eb[j] + ea[ig] - 12;
-                    mout[(size_t)j * ncouls + ig] = cplx{ldexp(m_re[sj][si][r], sc), ldexp(m_im[sj][si][r], sc)};
+                    double sr, sm;
+                    if constexpr (T::ONE_CHUNK) {
+                        fold(sj, si, r, sr, sm);
+                    } else {
+                        sr = m_re[(sj * SI + si) * 4 + r];
+                        sm = m_im[(sj * SI + si) * 4 + r];
+                    }
+                    mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc), ldexp(sm, sc)};
                 }
             }
 }
@@ -305,13 +383,14 @@ hipError_t launch_ozaki_s(const DevProblem &p, int layout, const cplx *bg, void 
     int *ea = reinterpret_cast<int *>(w + L.ea);
     D *bd = reinterpret_cast<D *>(w + L.bd);
     D *ad = reinterpret_cast<D *>(w + L.ad);
-    hipLaunchKernelGGL(k_oz_colexp, dim3((L.ngp_pad + 255) / 256), dim3(256), 0, s, bg, p.ngpown, p.nv, p.nb,
-                       p.ngpown, L.ngp_pad, eb);
-    hipLaunchKernelGGL(k_oz_colexp, dim3((L.nc_pad + 255) / 256), dim3(256), 0, s, p.aqsn, p.ncouls, p.nv, p.nb,
-                       p.ncouls, L.nc_pad, ea);
-    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.ngp_pad + 255) / 256, L.kpad / 8), dim3(256), 0, s, bg,
+    constexpr int KP = 16 / (int)sizeof(D);
+    hipLaunchKernelGGL(k_oz_colexp, dim3(L.ngp_pad / 64), dim3(256), 0, s, bg, p.ngpown, p.nv, p.nb, p.ngpown,
+                       L.ngp_pad, eb);
+    hipLaunchKernelGGL(k_oz_colexp, dim3(L.nc_pad / 64), dim3(256), 0, s, p.aqsn, p.ncouls, p.nv, p.nb, p.ncouls,
+                       L.nc_pad, ea);
+    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.ngp_pad + 255) / 256, L.kpad / KP), dim3(256), 0, s, bg,
                        p.ngpown, p.nv, L.K, p.ngpown, L.ngp_pad, L.kpad, eb, bd);
-    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.nc_pad + 255) / 256, L.kpad / 8), dim3(256), 0, s, p.aqsn,
+    hipLaunchKernelGGL((k_oz_split<S, D>), dim3((L.nc_pad + 255) / 256, L.kpad / KP), dim3(256), 0, s, p.aqsn,
                        p.ncouls, p.nv, L.K, p.ncouls, L.nc_pad, L.kpad, ea, ad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
