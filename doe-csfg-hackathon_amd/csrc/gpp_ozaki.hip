@@ -82,10 +82,17 @@ template <> struct OzDigit<int8_t> {
     {
         return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
     }
+    // Byte-wise two's complement negation, four bytes per dword without
+    // carries between bytes: -x = (~x) + 1 per byte.
     static __device__ __forceinline__ frag neg(frag x)
     {
-        v16c b = __builtin_bit_cast(v16c, x);
-        return __builtin_bit_cast(v4i, -b);
+        frag r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const unsigned t = ~(unsigned)x[i];
+            r[i] = (int)(((t & 0x7f7f7f7fu) + 0x01010101u) ^ (t & 0x80808080u));
+        }
+        return r;
     }
 };
 
