@@ -192,20 +192,18 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
     const int fo_re = 16 * ((lane / 16) ^ (lane & 7));
     const int fo_im = 16 * ((4 + lane / 16) ^ (lane & 7));
 
-    // Level accumulators; with chunked exact sums (FP16) also the FP64 running
-    // sums of the chunks (with INT8 the levels are exact over the whole band
-    // range and are folded once, in the epilogue).
+    // Level accumulators.  With INT8 the levels are exact over the whole band
+    // range and are folded to FP64 once, in the epilogue; with chunked exact
+    // sums (FP16) the FP64 running sums of the chunks are kept, unscaled, in
+    // the thread's own output elements of mout (registers are the limit here)
+    // and scaled after the last chunk.
     acc_t acc_re[S][SJ][SI], acc_im[S][SJ][SI];
-    constexpr int NM = T::ONE_CHUNK ? 1 : SJ * SI * 4;
-    double m_re[NM], m_im[NM];
 #pragma unroll
     for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
         for (int si = 0; si < SI; ++si)
 #pragma unroll
             for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < NM; ++i) m_re[i] = m_im[i] = 0.0;
     // Horner over the levels (smallest first) of result r of sub-tile (sj, si).
     auto fold = [&](int sj, int si, int r, double &sr, double &sm) {
         sr = 0.0;
@@ -217,19 +215,33 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
         }
     };
 
-    // Fold the level accumulators into the FP64 sums (chunked form only).
-    auto flush = [&]() {
+    // Fold the level accumulators into the FP64 sums (chunked form only):
+    // first chunk stores, later chunks add, the last one also scales.
+    auto flush = [&](bool first, bool last) {
 #pragma unroll
         for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
             for (int si = 0; si < SI; ++si)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    double sr, sm;
-                    fold(sj, si, r, sr, sm);
-                    const int i = T::ONE_CHUNK ? 0 : (sj * SI + si) * 4 + r;
-                    m_re[i] += sr;
-                    m_im[i] += sm;
+                    const int j = j0 + wj + 16 * sj + 4 * (lane / 16) + r;
+                    const int ig = i0 + wi + 16 * si + lane % 16;
+                    if (j < ngpown && ig < ncouls) {
+                        double sr, sm;
+                        fold(sj, si, r, sr, sm);
+                        cplx *o = &mout[(size_t)j * ncouls + ig];
+                        if (!first) {
+                            const cplx prev = *o;
+                            sr = prev.re + sr;
+                            sm = prev.im + sm;
+                        }
+                        if (last) {
+                            const int sc = eb[j] + ea[ig] - 12;
+                            sr = ldexp(sr, sc);
+                            sm = ldexp(sm, sc);
+                        }
+                        *o = cplx{sr, sm};
+                    }
                 }
 #pragma unroll
         for (int sj = 0; sj < SJ; ++sj)
@@ -327,35 +339,30 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
             }
         }
         __syncthreads();
+        if (k0 + KS < kpad) store_stage();  // the staged registers die before the flush
         if constexpr (!T::ONE_CHUNK) {
-            if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush();
+            if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush(k0 + KS <= T::CHUNK, k0 + KS >= kpad);
         }
-        if (k0 + KS < kpad) {
-            store_stage();
-            __syncthreads();
-        }
+        if (k0 + KS < kpad) __syncthreads();
     }
 
+    if constexpr (T::ONE_CHUNK) {
 #pragma unroll
-    for (int sj = 0; sj < SJ; ++sj)
+        for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
-        for (int si = 0; si < SI; ++si)
+            for (int si = 0; si < SI; ++si)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int j = j0 + wj + 16 * sj + 4 * (lane / 16) + r;
-                const int ig = i0 + wi + 16 * si + lane % 16;
-                if (j < ngpown && ig < ncouls) {
-                    const int sc = eb[j] + ea[ig] - 12;
-                    double sr, sm;
-                    if constexpr (T::ONE_CHUNK) {
+                for (int r = 0; r < 4; ++r) {
+                    const int j = j0 + wj + 16 * sj + 4 * (lane / 16) + r;
+                    const int ig = i0 + wi + 16 * si + lane % 16;
+                    if (j < ngpown && ig < ncouls) {
+                        const int sc = eb[j] + ea[ig] - 12;
+                        double sr, sm;
                         fold(sj, si, r, sr, sm);
-                    } else {
-                        sr = m_re[(sj * SI + si) * 4 + r];
-                        sm = m_im[(sj * SI + si) * 4 + r];
+                        mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc), ldexp(sm, sc)};
                     }
-                    mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc), ldexp(sm, sc)};
                 }
-            }
+    }
 }
 
 struct OzLayout {
