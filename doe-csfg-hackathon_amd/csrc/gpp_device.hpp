@@ -96,6 +96,18 @@ __device__ __forceinline__ cplx cfma(cplx a, cplx b, cplx acc)
                 __builtin_fma(a.re, b.im, __builtin_fma(a.im, b.re, acc.im))};
 }
 
+// XCD-aware tile order.  The dispatcher hands workgroup b (linear id) to XCD
+// b % 8, each XCD with its own L2; tile t of nt is given to the workgroup such
+// that every XCD walks a contiguous range of tiles, so neighbouring tiles
+// (which share an operand tile) meet in one L2.  A bijection on [0, nt).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nt)
+{
+    const int q = nt / kXcds, r = nt % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // Wave-wide sum by xor butterfly.  Every lane ends with the same, bitwise
 // reproducible value (a+b == b+a in IEEE arithmetic).
 __device__ __forceinline__ double wave_sum(double v)

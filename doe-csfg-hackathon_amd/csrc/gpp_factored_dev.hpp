@@ -22,18 +22,6 @@ constexpr int kKS = 8;             // bands per LDS stage of the FP64 Gram tile 
 constexpr int kWJ = 64, kWI = 32;  // FP64 Gram wave tile: 64 G' columns x 32 G-vectors
 constexpr int kRedThreads = 256;   // pole-reduce and final-reduce blocks
 
-// XCD-aware tile order.  The dispatcher hands workgroup b (linear id) to XCD
-// b % 8, each XCD with its own L2; tile t of nt is given to the workgroup such
-// that every XCD walks a contiguous range of tiles, so neighbouring tiles
-// (which share an operand tile) meet in one L2.  A bijection on [0, nt).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int nt)
-{
-    const int q = nt / kXcds, r = nt % kXcds;
-    const int x = b % kXcds, i = b / kXcds;
-    return x * q + (x < r ? x : r) + i;
-}
-
 // FP64 Gram block tile made of WJN x WIN waves.
 template <int WJN, int WIN> struct GramCfg {
     static constexpr int TJ = kWJ * WJN, TI = kWI * WIN, NT = 64 * WJN * WIN;

@@ -169,7 +169,8 @@ __global__ __launch_bounds__(256) void k_oz_split(const cplx *__restrict__ src, 
     }
 }
 
-// Mu tile from the digit planes.  Grid (ngpown_pad/64, ncouls_pad/64), 64*NWJ*NWI threads.
+// Mu tile from the digit planes.  1-D grid of (ngpown_pad/64) x (ncouls_pad/64)
+// tiles, 64*NWJ*NWI threads.
 template <int S, typename D, int LAYOUT>
 __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad,
                                                             const int *__restrict__ eb, const int *__restrict__ ea,
@@ -187,7 +188,11 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wj = (wave / W::NWI) * (16 * SJ), wi = (wave % W::NWI) * (16 * SI);
-    const int j0 = blockIdx.x * kOzTile, i0 = blockIdx.y * kOzTile;
+    // 1-D grid, tiles in XCD-aware order with G' tiles fastest: the blocks that
+    // share one G-vector tile's digit planes run on one XCD and share its L2.
+    const int ntj = ngp_pad / kOzTile;
+    const int tile = xcd_tile(blockIdx.x, ntj * (nc_pad / kOzTile));
+    const int j0 = (tile % ntj) * kOzTile, i0 = (tile / ntj) * kOzTile;
     // Fragment slots: every row this lane reads is = lane (mod 8).
     const int fo_re = 16 * ((lane / 16) ^ (lane & 7));
     const int fo_im = 16 * ((4 + lane / 16) ^ (lane & 7));
@@ -408,7 +413,7 @@ hipError_t launch_ozaki_s(const DevProblem &p, int layout, const cplx *bg, void 
                        p.ncouls, p.nv, L.K, p.ncouls, L.nc_pad, L.kpad, ea, ad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 grid(L.ngp_pad / kOzTile, L.nc_pad / kOzTile);
+    const dim3 grid((unsigned)(L.ngp_pad / kOzTile) * (unsigned)(L.nc_pad / kOzTile));
     if (layout == 1)
         hipLaunchKernelGGL((k_gram_ozaki<S, D, 1>), grid, dim3(64 * OzWaves<1>::NWJ * OzWaves<1>::NWI), 0, s, bd, ad,
                            eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, mout);
