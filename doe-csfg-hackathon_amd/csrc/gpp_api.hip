@@ -183,7 +183,8 @@ struct gpp_ctx {
     gpp::StageTimer timer;
     bool timed = false;
     int ozaki_slices = 6;
-    int ozaki_layout = 1;
+    int ozaki_layout = 2;
+    int ozaki_group = 0;
     int gram_tile = 0;
     int red_chunk = 8192;
     int red_fc = 8;
@@ -247,6 +248,7 @@ gpp::GramKind gram_kind(const gpp_ctx *ctx, gpp_form form)
     if (form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) k.ozaki_slices = ctx->ozaki_slices;
     k.int8 = form == GPP_FORM_FACTORED_OZAKI_I8;
     k.layout = ctx->ozaki_layout;
+    k.group = ctx->ozaki_group;
     k.tile = ctx->gram_tile;
     k.red_chunk = ctx->red_chunk;
     k.red_fc = ctx->red_fc;
@@ -764,8 +766,13 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         ctx->ozaki_slices = value;
         return GPP_OK;
     case GPP_OPT_OZAKI_LAYOUT:
-        if (value < 0 || value > 1) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0 or 1, got %d", value);
+        if (value < 0 || value > 2) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0..2, got %d", value);
         ctx->ozaki_layout = value;
+        return GPP_OK;
+    case GPP_OPT_OZAKI_GROUP:
+        if (value < 0 || value > (1 << 16))
+            return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki group must be in [0, 65536], got %d", value);
+        ctx->ozaki_group = value;
         return GPP_OK;
     case GPP_OPT_GRAM_TILE:
         if (value < 0 || value >= gpp::kGramTiles)
@@ -801,6 +808,7 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
     switch (option) {
     case GPP_OPT_OZAKI_SLICES: *value = ctx->ozaki_slices; return GPP_OK;
     case GPP_OPT_OZAKI_LAYOUT: *value = ctx->ozaki_layout; return GPP_OK;
+    case GPP_OPT_OZAKI_GROUP: *value = ctx->ozaki_group; return GPP_OK;
     case GPP_OPT_GRAM_TILE: *value = ctx->gram_tile; return GPP_OK;
     case GPP_OPT_REDUCE_CHUNK: *value = ctx->red_chunk; return GPP_OK;
     case GPP_OPT_REDUCE_FREQS: *value = ctx->red_fc; return GPP_OK;

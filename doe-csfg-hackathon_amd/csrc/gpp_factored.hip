@@ -275,7 +275,7 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     GPP_TRY(launch_achstemp(p, part_achs, s));
     if (timer) timer->mark(s, "factored_gather_achstemp");
     if (kind.ozaki_slices) {
-        GPP_TRY(launch_gram_ozaki(p, kind.ozaki_slices, kind.int8, kind.layout, bg, w + L.total, mu, s));
+        GPP_TRY(launch_gram_ozaki(p, kind.ozaki_slices, kind.int8, kind.layout, kind.group, bg, w + L.total, mu, s));
         if (timer) timer->mark(s, kind.int8 ? "factored_gram_ozaki_i8" : "factored_gram_ozaki");
     } else {
         GPP_TRY(launch_gram_f64(p, kind.tile, bg, mu, s));

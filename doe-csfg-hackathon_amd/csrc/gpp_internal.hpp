@@ -106,7 +106,8 @@ hipError_t launch_fill_seeded(cplx *dst, size_t n, uint64_t seed, uint32_t array
 struct GramKind {
     int ozaki_slices = 0;  // 0: FP64 MFMA; 3..6: Ozaki split with that many slices
     bool int8 = false;     // Ozaki digits in INT8 (else FP16)
-    int layout = 1;        // Ozaki wave layout (0: 2x2 waves of 32x32, 1: 2x4 waves of 32x16)
+    int layout = 2;        // Ozaki block layout (0, 1: 64x64 tiles, registers staged; 2: 64x32, direct loads)
+    int group = 0;         // Ozaki G' tiles per tile group (0: sized for the L2)
     int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
     int red_chunk = 8192;  // G-vectors per pole-reduce block
     int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
@@ -138,8 +139,8 @@ hipError_t run_factored_bucketed(const DevProblem &p, GramKind kind, const Bucke
 // Ozaki Gram product (gpp_ozaki.hip): Mu = conj(Bg)^T aqsn over bands
 // [nv, nb), written to mout [ngpown][ncouls].
 size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8);
-hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, const cplx *bg, void *work,
-                             cplx *mout, hipStream_t s);
+hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, int group, const cplx *bg,
+                             void *work, cplx *mout, hipStream_t s);
 
 // ---- batched factored form (gpp_batch.hip) --------------------------------
 struct GramItem {  // one Gram tile of one instance

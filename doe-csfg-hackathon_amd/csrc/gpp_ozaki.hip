@@ -42,7 +42,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef signed char v16c __attribute__((ext_vector_type(16)));
 
-constexpr int kOzTile = 64;     // block tile: 64 G' columns x 64 G-vectors
+constexpr int kOzTile = 64;     // padding of both operands' columns
 // LDS: per digit plane p and tile row (a column j or ig), one 128-byte row
 // holding the stage's real-part digits (16-byte units 0-3) and imaginary-part
 // digits (units 4-7); logical unit u of row r sits in slot u ^ (r & 7), which
@@ -50,13 +50,29 @@ constexpr int kOzTile = 64;     // block tile: 64 G' columns x 64 G-vectors
 // LDS bank conflicts on gfx950 (a padded 80-byte pitch is not).
 constexpr int kOzRowBytes = 128;
 
-// Wave layouts of the 64 x 64 block tile: NWJ x NWI waves, each owning
-// SJ x SI sub-tiles of 16 x 16 (NWJ*SJ*16 = NWI*SI*16 = 64).
-//   layout 0: 2 x 2 waves of 32 x 32 (one wave per SIMD, 512 registers each)
-//   layout 1: 2 x 4 waves of 32 x 16 (two waves per SIMD, fewer accumulators)
+// Block layouts: NWJ x NWI waves, each owning SJ x SI sub-tiles of 16 x 16,
+// so a block tile is TJ = 16 NWJ SJ G' columns by TI = 16 NWI SI G-vectors.
+//   layout 0: 64 x 64, 2 x 2 waves of 32 x 32 (one wave per SIMD, 512
+//             registers each), stages staged through registers
+//   layout 1: 64 x 64, 2 x 4 waves of 32 x 16 (two waves per SIMD), staged
+//             through registers
+//   layout 2: 64 x 32, 2 x 2 waves of 32 x 16, stages loaded straight from
+//             memory into LDS (global_load_lds_dwordx4, no staging registers);
+//             two blocks share a CU, so one block's loads and barriers overlap
+//             the other block's MFMAs.
 template <int L> struct OzWaves;
-template <> struct OzWaves<0> { static constexpr int NWJ = 2, NWI = 2, SJ = 2, SI = 2; };
-template <> struct OzWaves<1> { static constexpr int NWJ = 2, NWI = 4, SJ = 2, SI = 1; };
+template <> struct OzWaves<0> {
+    static constexpr int NWJ = 2, NWI = 2, SJ = 2, SI = 2, BLOCKS_PER_CU = 1;
+    static constexpr bool DMA = false;
+};
+template <> struct OzWaves<1> {
+    static constexpr int NWJ = 2, NWI = 4, SJ = 2, SI = 1, BLOCKS_PER_CU = 1;
+    static constexpr bool DMA = false;
+};
+template <> struct OzWaves<2> {
+    static constexpr int NWJ = 2, NWI = 2, SJ = 2, SI = 1, BLOCKS_PER_CU = 2;
+    static constexpr bool DMA = true;
+};
 
 // Digit-type traits: FP16 digits on f32_16x16x32_f16 with exact FP32 sums over
 // chunks of 256 bands; INT8 digits on i32_16x16x64_i8 with exact INT32 sums.
@@ -95,6 +111,13 @@ template <> struct OzDigit<int8_t> {
         return r;
     }
 };
+
+// 16 bytes per lane from global memory straight into LDS: the wave writes
+// 1 KB at lds_wave (wave-uniform) + 16 * lane.
+__device__ __forceinline__ void load_lds16(const void *src, void *lds_wave)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds_wave, 16, 0, 0);
+}
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 __host__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -169,30 +192,39 @@ __global__ __launch_bounds__(256) void k_oz_split(const cplx *__restrict__ src, 
     }
 }
 
-// Mu tile from the digit planes.  1-D grid of (ngpown_pad/64) x (ncouls_pad/64)
-// tiles, 64*NWJ*NWI threads.
+// Mu tiles from the digit planes.  1-D grid of (ngpown_pad/TJ) x (ncouls_pad/TI)
+// tiles, 64*NWJ*NWI threads.  Tile order: groups of gj G' tiles; within a
+// group, G-vector tiles in turn with the group's G' tiles fastest; the order
+// is split into one contiguous range per XCD.  The blocks of an XCD then
+// share the digit planes of a few G' tiles (kept in its L2) and stream the
+// G-vector tiles once per group.
 template <int S, typename D, int LAYOUT>
-__global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad,
-                                                            const int *__restrict__ eb, const int *__restrict__ ea,
-                                                            int ngpown, int ncouls, int ngp_pad, int nc_pad, int kpad,
-                                                            cplx *__restrict__ mout)
+__global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI, OzWaves<LAYOUT>::BLOCKS_PER_CU)
+void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad, const int *__restrict__ eb,
+                  const int *__restrict__ ea, int ngpown, int ncouls, int ngp_pad, int nc_pad, int kpad, int gj,
+                  cplx *__restrict__ mout)
 {
     typedef OzDigit<D> T;
     typedef OzWaves<LAYOUT> W;
     constexpr int SJ = W::SJ, SI = W::SI, NTH = 64 * W::NWJ * W::NWI;
+    constexpr int TJ = 16 * W::NWJ * SJ, TI = 16 * W::NWI * SI;
     typedef typename T::frag frag;
     typedef typename T::acc acc_t;
     constexpr int KS = T::KS;
-    __shared__ __attribute__((aligned(16))) unsigned char sB[S][kOzTile][kOzRowBytes];
-    __shared__ __attribute__((aligned(16))) unsigned char sA[S][kOzTile][kOzRowBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[S][TJ][kOzRowBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sA[S][TI][kOzRowBytes];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wj = (wave / W::NWI) * (16 * SJ), wi = (wave % W::NWI) * (16 * SI);
-    // 1-D grid, tiles in XCD-aware order with G' tiles fastest: the blocks that
-    // share one G-vector tile's digit planes run on one XCD and share its L2.
-    const int ntj = ngp_pad / kOzTile;
-    const int tile = xcd_tile(blockIdx.x, ntj * (nc_pad / kOzTile));
-    const int j0 = (tile % ntj) * kOzTile, i0 = (tile / ntj) * kOzTile;
+    int j0, i0;
+    {
+        const int ntj = ngp_pad / TJ, nti = nc_pad / TI;
+        const int t = xcd_tile(blockIdx.x, ntj * nti);
+        const int g = t / (gj * nti), rem = t - g * gj * nti;
+        const int gsz = min(gj, ntj - g * gj);
+        j0 = (g * gj + rem % gsz) * TJ;
+        i0 = (rem / gsz) * TI;
+    }
     // Fragment slots: every row this lane reads is = lane (mod 8).
     const int fo_re = 16 * ((lane / 16) ^ (lane & 7));
     const int fo_im = 16 * ((4 + lane / 16) ^ (lane & 7));
@@ -262,45 +294,63 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
     const int npieces = kpad / KP;
     const unsigned char *gb = reinterpret_cast<const unsigned char *>(bd);
     const unsigned char *ga = reinterpret_cast<const unsigned char *>(ad);
-    // 16-byte staging units per stage: S planes x 64 rows x 8 per side.  The
-    // next stage's units are loaded into registers while the current stage
-    // computes, then stored to LDS between two barriers.
-    constexpr int UNITS = S * kOzTile * 8;
-    constexpr int PER_T = 2 * UNITS / NTH;
-    static_assert(PER_T * NTH == 2 * UNITS, "stage must split evenly");
-    v4i stage_reg[PER_T];
+    // 16-byte units per stage: S planes x TJ rows x 8 (G' side), then S planes
+    // x TI rows x 8 (G-vector side); unit v of a side lands at byte 16 v of its
+    // LDS array ([plane][row][slot]).
+    constexpr int UB = S * TJ * 8, UA = S * TI * 8;
+    constexpr int PER_T = (UB + UA) / NTH;
+    static_assert(PER_T * NTH == UB + UA && UB % kWave == 0, "stage must split evenly, waves within a side");
+    // Global address of unit u of the stage starting at piece kq0.
+    auto unit_src = [&](int u, int kq0) -> const unsigned char * {
+        const bool side = u >= UB;
+        const int v = side ? u - UB : u;
+        const int rows = side ? TI : TJ;
+        const int slot = v % 8, row = (v / 8) % rows, p = v / (8 * rows);
+        const int lu = slot ^ (row & 7);  // logical unit: part = lu / 4, 16-byte piece lu % 4
+        const int npad = side ? nc_pad : ngp_pad;
+        const int col = (side ? i0 : j0) + row;
+        return (side ? ga : gb) + (((size_t)(p * 2 + lu / 4) * npieces + kq0 + lu % 4) * npad + col) * 16;
+    };
+    // Register staging (layouts 0, 1): the next stage's units are loaded into
+    // registers while the current stage computes, then stored to LDS between
+    // two barriers.
+    v4i stage_reg[W::DMA ? 1 : PER_T];
     auto load_stage = [&](int k0) {
-        const int kq0 = k0 / KP;  // a stage is 4 pieces
 #pragma unroll
-        for (int q = 0; q < PER_T; ++q) {
-            const int u = threadIdx.x + q * NTH;
-            const int side = u / UNITS;
-            const int v = u % UNITS;
-            const int slot = v % 8, row = (v / 8) % kOzTile, p = v / (8 * kOzTile);
-            const int lu = slot ^ (row & 7);  // logical unit: part = lu / 4, 16-byte piece lu % 4
-            const unsigned char *src = side ? ga : gb;
-            const int npad = side ? nc_pad : ngp_pad;
-            const int col = (side ? i0 : j0) + row;
-            stage_reg[q] = *reinterpret_cast<const v4i *>(
-                src + (((size_t)(p * 2 + lu / 4) * npieces + kq0 + lu % 4) * npad + col) * 16);
-        }
+        for (int q = 0; q < PER_T; ++q)
+            stage_reg[q] = *reinterpret_cast<const v4i *>(unit_src(threadIdx.x + q * NTH, k0 / KP));
     };
     auto store_stage = [&]() {
 #pragma unroll
         for (int q = 0; q < PER_T; ++q) {
             const int u = threadIdx.x + q * NTH;
-            const int side = u / UNITS;
-            const int v = u % UNITS;
-            const int slot = v % 8, row = (v / 8) % kOzTile, p = v / (8 * kOzTile);
-            unsigned char *dst = side ? &sA[p][row][slot * 16] : &sB[p][row][slot * 16];
+            unsigned char *dst = u >= UB ? &sA[0][0][0] + 16 * (u - UB) : &sB[0][0][0] + 16 * u;
             *reinterpret_cast<v4i *>(dst) = stage_reg[q];
         }
     };
-    load_stage(0);
-    store_stage();
+    // Direct loads (layout 2): each wave instruction fills 1 KB of LDS, 16
+    // bytes per lane in lane order, from the addresses its lanes give.
+    auto dma_stage = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER_T; ++q) {
+            const int u = threadIdx.x + q * NTH;
+            const int ub = (threadIdx.x & ~(kWave - 1)) + q * NTH;  // the wave's first unit
+            unsigned char *dst = ub >= UB ? &sA[0][0][0] + 16 * (ub - UB) : &sB[0][0][0] + 16 * ub;
+            load_lds16(unit_src(u, k0 / KP), dst);
+        }
+    };
+    if constexpr (W::DMA) {
+        dma_stage(0);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        load_stage(0);
+        store_stage();
+    }
     __syncthreads();
     for (int k0 = 0; k0 < kpad; k0 += KS) {
-        if (k0 + KS < kpad) load_stage(k0 + KS);
+        if constexpr (!W::DMA) {
+            if (k0 + KS < kpad) load_stage(k0 + KS);
+        }
         // G' side fragments of digit p are held across its pairs; the G-vector
         // side fragments of digit q are read from LDS for each pair (fewer
         // live registers, which leaves room for the next stage's loads).
@@ -343,12 +393,28 @@ __global__ __launch_bounds__(64 * OzWaves<LAYOUT>::NWJ * OzWaves<LAYOUT>::NWI) v
                     for (int si = 0; si < SI; ++si) acc_im[t][sj][si] = T::mfma(bn[sj], fa[0][si], acc_im[t][sj][si]);
             }
         }
-        __syncthreads();
-        if (k0 + KS < kpad) store_stage();  // the staged registers die before the flush
-        if constexpr (!T::ONE_CHUNK) {
-            if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush(k0 + KS <= T::CHUNK, k0 + KS >= kpad);
+        if constexpr (W::DMA) {
+            // All waves are done with the stage before it is overwritten; the
+            // flush's memory traffic overlaps the loads.
+            if (k0 + KS < kpad) {
+                __syncthreads();
+                dma_stage(k0 + KS);
+            }
+            if constexpr (!T::ONE_CHUNK) {
+                if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush(k0 + KS <= T::CHUNK, k0 + KS >= kpad);
+            }
+            if (k0 + KS < kpad) {
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+        } else {
+            __syncthreads();
+            if (k0 + KS < kpad) store_stage();  // the staged registers die before the flush
+            if constexpr (!T::ONE_CHUNK) {
+                if ((k0 + KS) % T::CHUNK == 0 || k0 + KS >= kpad) flush(k0 + KS <= T::CHUNK, k0 + KS >= kpad);
+            }
+            if (k0 + KS < kpad) __syncthreads();
         }
-        if (k0 + KS < kpad) __syncthreads();
     }
 
     if constexpr (T::ONE_CHUNK) {
@@ -393,8 +459,28 @@ OzLayout oz_layout(const DevProblem &p, int slices, bool int8)
     return L;
 }
 
+template <int S, typename D, int LAYOUT>
+void launch_gram_tiles(const OzLayout &L, const DevProblem &p, const D *bd, const D *ad, const int *eb, const int *ea,
+                       int group, cplx *mout, hipStream_t s)
+{
+    typedef OzWaves<LAYOUT> W;
+    constexpr int TJ = 16 * W::NWJ * W::SJ, TI = 16 * W::NWI * W::SI;
+    const int ntj = L.ngp_pad / TJ;
+    // Default group: G' tiles whose digit planes (S x 2 planes x TJ x kpad
+    // digits each) fill about 2.5 MB of an XCD's 4 MB L2.
+    if (group <= 0) {
+        const size_t tile_bytes = (size_t)S * 2 * TJ * L.kpad * sizeof(D);
+        group = (int)((size_t)(5 << 19) / tile_bytes);
+    }
+    group = group < 1 ? 1 : group > ntj ? ntj : group;
+    const dim3 grid((unsigned)ntj * (unsigned)(L.nc_pad / TI));
+    hipLaunchKernelGGL((k_gram_ozaki<S, D, LAYOUT>), grid, dim3(64 * W::NWJ * W::NWI), 0, s, bd, ad, eb, ea, p.ngpown,
+                       p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, group, mout);
+}
+
 template <int S, typename D>
-hipError_t launch_ozaki_s(const DevProblem &p, int layout, const cplx *bg, void *work, cplx *mout, hipStream_t s)
+hipError_t launch_ozaki_s(const DevProblem &p, int layout, int group, const cplx *bg, void *work, cplx *mout,
+                          hipStream_t s)
 {
     const OzLayout L = oz_layout(p, S, sizeof(D) == 1);
     char *w = static_cast<char *>(work);
@@ -413,25 +499,23 @@ hipError_t launch_ozaki_s(const DevProblem &p, int layout, const cplx *bg, void 
                        p.ncouls, p.nv, L.K, p.ncouls, L.nc_pad, L.kpad, ea, ad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 grid((unsigned)(L.ngp_pad / kOzTile) * (unsigned)(L.nc_pad / kOzTile));
-    if (layout == 1)
-        hipLaunchKernelGGL((k_gram_ozaki<S, D, 1>), grid, dim3(64 * OzWaves<1>::NWJ * OzWaves<1>::NWI), 0, s, bd, ad,
-                           eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, mout);
-    else
-        hipLaunchKernelGGL((k_gram_ozaki<S, D, 0>), grid, dim3(64 * OzWaves<0>::NWJ * OzWaves<0>::NWI), 0, s, bd, ad,
-                           eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, mout);
+    switch (layout) {
+    case 0: launch_gram_tiles<S, D, 0>(L, p, bd, ad, eb, ea, group, mout, s); break;
+    case 1: launch_gram_tiles<S, D, 1>(L, p, bd, ad, eb, ea, group, mout, s); break;
+    default: launch_gram_tiles<S, D, 2>(L, p, bd, ad, eb, ea, group, mout, s); break;
+    }
     return hipGetLastError();
 }
 
 template <typename D>
-hipError_t launch_ozaki_d(const DevProblem &p, int slices, int layout, const cplx *bg, void *work, cplx *mout,
-                          hipStream_t s)
+hipError_t launch_ozaki_d(const DevProblem &p, int slices, int layout, int group, const cplx *bg, void *work,
+                          cplx *mout, hipStream_t s)
 {
     switch (slices) {
-    case 3: return launch_ozaki_s<3, D>(p, layout, bg, work, mout, s);
-    case 4: return launch_ozaki_s<4, D>(p, layout, bg, work, mout, s);
-    case 5: return launch_ozaki_s<5, D>(p, layout, bg, work, mout, s);
-    case 6: return launch_ozaki_s<6, D>(p, layout, bg, work, mout, s);
+    case 3: return launch_ozaki_s<3, D>(p, layout, group, bg, work, mout, s);
+    case 4: return launch_ozaki_s<4, D>(p, layout, group, bg, work, mout, s);
+    case 5: return launch_ozaki_s<5, D>(p, layout, group, bg, work, mout, s);
+    case 6: return launch_ozaki_s<6, D>(p, layout, group, bg, work, mout, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -440,12 +524,12 @@ hipError_t launch_ozaki_d(const DevProblem &p, int slices, int layout, const cpl
 
 size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8) { return oz_layout(p, slices, int8).total; }
 
-hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, const cplx *bg, void *work,
-                             cplx *mout, hipStream_t s)
+hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, int group, const cplx *bg,
+                             void *work, cplx *mout, hipStream_t s)
 {
     if (int8 && (p.nb - p.nv) >= 43690) return hipErrorInvalidValue;  // INT32 exactness bound
-    return int8 ? launch_ozaki_d<int8_t>(p, slices, layout, bg, work, mout, s)
-                : launch_ozaki_d<_Float16>(p, slices, layout, bg, work, mout, s);
+    return int8 ? launch_ozaki_d<int8_t>(p, slices, layout, group, bg, work, mout, s)
+                : launch_ozaki_d<_Float16>(p, slices, layout, group, bg, work, mout, s);
 }
 
 }  // namespace gpp

@@ -102,9 +102,11 @@ typedef enum {
      * slice carries 7 bits; the split truncates at about 2^(-7 S) of the
      * column scale. */
     GPP_OPT_OZAKI_SLICES = 1,
-    /* Wave layout of the Ozaki Gram kernel's 64x64 block tile: 0 = 2x2 waves
-     * of 32x32 (one wave per SIMD), 1 = 2x4 waves of 32x16 (two per SIMD,
-     * default). */
+    /* Block layout of the Ozaki Gram kernel: 0 = 64x64 tiles, 2x2 waves of
+     * 32x32 (one wave per SIMD); 1 = 64x64 tiles, 2x4 waves of 32x16 (two per
+     * SIMD), both staging each band step through registers; 2 = 64x32 tiles,
+     * 2x2 waves of 32x16, band steps loaded straight into LDS, two blocks per
+     * compute unit (default). */
     GPP_OPT_OZAKI_LAYOUT = 2,
     /* FP64-MFMA Gram tile (G' columns x G-vectors): 0 = 128x128 (default),
      * 1 = 64x128, 2 = 128x64, 3 = 64x64. */
@@ -123,7 +125,11 @@ typedef enum {
      * GPP_OPT_REDUCE_FREQS, re-reading its G-vector range from the caches
      * between passes: 0 = all nfreq (default); k > 0 splits the frequencies
      * over ceil(nfreq / k) blocks per range. */
-    GPP_OPT_REDUCE_FREQS_PER_BLOCK = 7
+    GPP_OPT_REDUCE_FREQS_PER_BLOCK = 7,
+    /* G' column tiles per tile group of the Ozaki Gram kernel: the blocks of
+     * a group share those tiles' digit planes in L2 and stream the G-vector
+     * digit planes once per group.  0 = sized for the L2 (default). */
+    GPP_OPT_OZAKI_GROUP = 8
 } gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).

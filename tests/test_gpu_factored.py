@@ -130,24 +130,32 @@ def test_ozaki_error_falls_with_slices(ctx, oracle, form):
         assert b < a / 16, errs
 
 
-def test_ozaki_layouts_and_digit_types_agree(ctx):
-    """Both wave layouts compute the same exact integer sums; FP16 and INT8
-    digits differ only in where FP64 rounding of the exact sums happens."""
+@pytest.mark.parametrize("nb", [30, 300])
+def test_ozaki_layouts_and_digit_types_agree(ctx, oracle, nb):
+    """All block layouts and tile orders compute the same exact integer sums
+    (bit-identical outputs); FP16 and INT8 digits differ only in where FP64
+    rounding of the exact sums happens.  300 bands: several band steps per
+    tile and two FP16 chunks."""
     import pygpp
-    arrs = gpp_inputs.near_pole_inputs(30, 700, 70, seed=22)
+    arrs = gpp_inputs.near_pole_inputs(nb, 700, 70, seed=22)
     ctx.upload(problem_of(arrs, 3), arrs)
     out = {}
     try:
-        for lay in (0, 1):
+        for lay, group in ((0, 0), (1, 0), (2, 0), (2, 1), (2, 5)):
             ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, lay)
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_GROUP, group)
             for form in ("factored-ozaki", "factored-ozaki-i8"):
-                out[(form, lay)] = ctx.sigma(form)
+                out[(form, lay, group)] = ctx.sigma(form)
     finally:
-        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 1)
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 2)
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_GROUP, 0)
     for form in ("factored-ozaki", "factored-ozaki-i8"):
-        for k in KEYS:
-            np.testing.assert_array_equal(out[(form, 0)][k], out[(form, 1)][k])
-    assert_close(out[("factored-ozaki", 1)], out[("factored-ozaki-i8", 1)], 1e-13, KEYS)
+        for key in [(0, 0), (1, 0), (2, 1), (2, 5)]:
+            for k in KEYS:
+                np.testing.assert_array_equal(out[(form,) + key][k], out[(form, 2, 0)][k])
+    assert_close(out[("factored-ozaki", 2, 0)], out[("factored-ozaki-i8", 2, 0)], 1e-13, KEYS)
+    ref = oracle.sigma(arrs, nvband=3, threads=8)
+    assert_close(out[("factored-ozaki-i8", 2, 0)], ref, TOL_FACTORED, KEYS)
 
 
 def test_ozaki_option_validation(ctx):
@@ -155,8 +163,12 @@ def test_ozaki_option_validation(ctx):
     for bad in (2, 7):
         with pytest.raises(pygpp.GppError):
             ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, bad)
-    with pytest.raises(pygpp.GppError):
-        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 5)
+    for opt, bad in ((pygpp.GPP_OPT_OZAKI_LAYOUT, 3), (pygpp.GPP_OPT_OZAKI_LAYOUT, -1),
+                     (pygpp.GPP_OPT_OZAKI_GROUP, -1)):
+        with pytest.raises(pygpp.GppError):
+            ctx.set_option(opt, bad)
+    assert ctx.get_option(pygpp.GPP_OPT_OZAKI_LAYOUT) == 2
+    assert ctx.get_option(pygpp.GPP_OPT_OZAKI_GROUP) == 0
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
 
 
