@@ -102,11 +102,14 @@ typedef enum {
      * slice carries 7 bits; the split truncates at about 2^(-7 S) of the
      * column scale. */
     GPP_OPT_OZAKI_SLICES = 1,
-    /* Block layout of the Ozaki Gram kernel: 0 = 64x64 tiles, 2x2 waves of
-     * 32x32 (one wave per SIMD); 1 = 64x64 tiles, 2x4 waves of 32x16 (two per
-     * SIMD), both staging each band step through registers; 2 = 64x32 tiles,
-     * 2x2 waves of 32x16, band steps loaded straight into LDS, two blocks per
-     * compute unit (default). */
+    /* Block layout of the Ozaki Gram kernel (all give bit-identical results):
+     * 0 = 64x64 tiles, 2x2 waves of 32x32 (one wave per SIMD); 1 = 64x64
+     * tiles, 2x4 waves of 32x16 (two per SIMD), both on 16x16 MFMAs staging
+     * each band step through registers; 2 = 64x32 tiles, 2x2 waves of 32x16
+     * on 16x16 MFMAs, band steps loaded straight into LDS, two blocks per
+     * compute unit; 3 = 64x64 tiles, 2x2 waves of 32x32 on 32x32 MFMAs, band
+     * steps loaded straight into two to four LDS buffers ahead of use
+     * (default). */
     GPP_OPT_OZAKI_LAYOUT = 2,
     /* FP64-MFMA Gram tile (G' columns x G-vectors): 0 = 128x128 (default),
      * 1 = 64x128, 2 = 128x64, 3 = 64x64. */
