@@ -109,14 +109,15 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restr
                             mout);
 }
 
-// Grid (ngpown, nsplit, ceil(nfreq/FC)).
-template <int FC>
+// Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
+// compile time (0: any).
+template <int FC, int NV>
 __global__ __launch_bounds__(kRedThreads) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
     const int iw_end = iw_begin + a.fpb < a.nfreq ? iw_begin + a.fpb : a.nfreq;
-    pole_reduce_item<FC>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
+    pole_reduce_item<FC, NV>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach, const cplx *col_asx,
@@ -209,7 +210,12 @@ void launch_reduce_fc(ReduceArgs a, hipStream_t s)
 {
     if (a.fpb < FC) a.fpb = FC;
     dim3 grid(a.ngpown, a.nsplit, (a.nfreq + a.fpb - 1) / a.fpb);
-    hipLaunchKernelGGL((k_pole_reduce<FC>), grid, dim3(kRedThreads), 0, s, a);
+    switch (a.nv) {  // the occupied band counts of the common shapes
+    case 2: hipLaunchKernelGGL((k_pole_reduce<FC, 2>), grid, dim3(kRedThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_pole_reduce<FC, 4>), grid, dim3(kRedThreads), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_pole_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_pole_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, a); break;
+    }
 }
 
 #define GPP_TRY(x)                       \

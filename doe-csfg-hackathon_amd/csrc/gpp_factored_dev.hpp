@@ -172,13 +172,16 @@ struct ReduceArgs {
 // -(I_eps wt / 2) / (wx + wt) and the accumulations use FMA.  sch = P in branch 1 (:37) shares the
 // unoccupied product.  Block-wide fixed-order sums; writes the column
 // partials (0.5*sum)*vcoul and sum*(occ*vcoul).  scratch: 4 entries.
-template <int FC>
+// NV > 0: the occupied band count is NV (== a.nv), known at compile time, so
+// the occupied loads are unrolled and issued together and conj(bg) of the
+// column is loaded once; NV == 0: any a.nv.
+template <int FC, int NV = 0>
 __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw_begin, int iw_end,
                                                  cplx *scratch)
 {
 #pragma clang fp contract(off)
     const int ncouls = a.ncouls;
-    const int nv = a.nv;
+    const int nv = NV > 0 ? NV : a.nv;
 
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
@@ -209,6 +212,7 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         cplx mo = czero(), om2 = czero(), mum = mu, qh = czero();
         double ie_cut = 0.0, wti2 = 0.0;
         if (nv > 0) {
+#pragma unroll
             for (int n1 = 0; n1 < nv; ++n1)  // occupied bands, in band order
                 mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
             om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
