@@ -78,7 +78,8 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_batch_gram(const BInst *__re
     }
 }
 
-template <int FC>
+// Reduce items of instances whose occupied band count is NV (0: any count).
+template <int FC, int NV>
 __global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__restrict__ insts,
                                                               const RedItem *__restrict__ items, int nitems,
                                                               int chunk)
@@ -94,8 +95,8 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__res
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
         ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
         ra.fpb = FC;
-        pole_reduce_item<FC>(ra, it.j, it.split, it.iw0, it.iw0 + FC < b.nfreq ? it.iw0 + FC : b.nfreq,
-                             scratch);
+        pole_reduce_item<FC, NV>(ra, it.j, it.split, it.iw0, it.iw0 + FC < b.nfreq ? it.iw0 + FC : b.nfreq,
+                                 scratch);
     }
 }
 
@@ -116,6 +117,11 @@ constexpr int gram_ti(int tile) { return tile == 0 || tile == 1 ? 128 : 64; }
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Reduce item classes: class c in 1..8 holds the instances with c occupied
+// bands (c >= 2 run kernels specialised on that count), class 0 the rest.
+constexpr int kRedClasses = 9;
+inline int red_class(int nv) { return nv >= 2 && nv <= 8 ? nv : 0; }
+
 BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes)
 {
     BatchPlan pl;
@@ -142,7 +148,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
     pl.tab_goff = off;  off += align256(sizeof(size_t) * (pl.n + 1));
     // work lists
     std::vector<std::pair<long, GramItem>> g;
-    std::vector<std::pair<long, RedItem>> r;
+    std::vector<std::pair<long, RedItem>> r[kRedClasses];
     std::vector<int2> a;
     const int tj = gram_tj(k.tile), ti = gram_ti(k.tile);
     for (int i = 0; i < pl.n; ++i) {
@@ -156,16 +162,20 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
             for (int j = 0; j < p.ngpown; ++j)
                 for (int sp = 0; sp < nsplit; ++sp) {
                     const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
-                    r.push_back({len * fc * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp, iw0}});
+                    r[red_class(p.nv)].push_back({len * fc * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp, iw0}});
                 }
         for (int n1 = 0; n1 < p.nb; ++n1) a.push_back(int2{i, n1});
     }
     auto by_cost = [](const auto &x, const auto &y) { return x.first > y.first; };
     std::stable_sort(g.begin(), g.end(), by_cost);
-    std::stable_sort(r.begin(), r.end(), by_cost);
     pl.gram_items.clear();
     for (auto &e : g) pl.gram_items.push_back(e.second);
-    for (auto &e : r) pl.red_items.push_back(e.second);
+    for (int c = 0; c < kRedClasses; ++c) {
+        std::stable_sort(r[c].begin(), r[c].end(), by_cost);
+        pl.red_class_start[c] = (int)pl.red_items.size();
+        for (auto &e : r[c]) pl.red_items.push_back(e.second);
+    }
+    pl.red_class_start[kRedClasses] = (int)pl.red_items.size();
     pl.ach_items = a;
     pl.tab_gram = off; off += align256(sizeof(GramItem) * (pl.gram_items.size() + 1));
     pl.tab_red = off;  off += align256(sizeof(RedItem) * (pl.red_items.size() + 1));
@@ -216,6 +226,21 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
     return hipStreamSynchronize(s);
 }
 
+template <int FC>
+void launch_batch_reduce(int cls, dim3 grid, hipStream_t s, const BInst *insts, const RedItem *ri, int nr, int chunk)
+{
+    switch (cls) {
+    case 2: hipLaunchKernelGGL((k_batch_reduce<FC, 2>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 3: hipLaunchKernelGGL((k_batch_reduce<FC, 3>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 4: hipLaunchKernelGGL((k_batch_reduce<FC, 4>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 5: hipLaunchKernelGGL((k_batch_reduce<FC, 5>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 6: hipLaunchKernelGGL((k_batch_reduce<FC, 6>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 7: hipLaunchKernelGGL((k_batch_reduce<FC, 7>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 8: hipLaunchKernelGGL((k_batch_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    default: hipLaunchKernelGGL((k_batch_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    }
+}
+
 // Launches the batch's stages; `which` selects stages (bit 0 gather+achstemp,
 // bit 1 Gram, bit 2 reduce, bit 3 final) so the autotuner can time one stage.
 hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, hipStream_t s, int which)
@@ -241,13 +266,17 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         }
     }
     if (which & 4) {
-        const RedItem *ri = reinterpret_cast<const RedItem *>(w + pl.tab_red);
-        const int nr = (int)pl.red_items.size();
-        const dim3 grid(std::max(1, std::min(G, nr)));
-        if (pl.fc == 4)
-            hipLaunchKernelGGL((k_batch_reduce<4>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, pl.kind.red_chunk);
-        else
-            hipLaunchKernelGGL((k_batch_reduce<8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, pl.kind.red_chunk);
+        // One persistent launch per occupied band class present.
+        for (int c = 0; c < kRedClasses; ++c) {
+            const int b0 = pl.red_class_start[c], nr = pl.red_class_start[c + 1] - b0;
+            if (nr == 0) continue;
+            const RedItem *ri = reinterpret_cast<const RedItem *>(w + pl.tab_red) + b0;
+            const dim3 grid(std::max(1, std::min(G, nr)));
+            if (pl.fc == 4)
+                launch_batch_reduce<4>(c, grid, s, insts, ri, nr, pl.kind.red_chunk);
+            else
+                launch_batch_reduce<8>(c, grid, s, insts, ri, nr, pl.kind.red_chunk);
+        }
     }
     if (which & 8)
         hipLaunchKernelGGL(k_batch_final, dim3(pl.n, pl.max_nfreq + 1), dim3(kRedThreads), 0, s, insts);

@@ -162,7 +162,8 @@ struct BatchPlan {
     GramKind kind;
     std::vector<InstOff> inst_off;
     std::vector<GramItem> gram_items;
-    std::vector<RedItem> red_items;
+    std::vector<RedItem> red_items;  // grouped by occupied band class (plan_batch), each class by cost
+    int red_class_start[10] = {};    // items of class c: [red_class_start[c], red_class_start[c + 1])
     std::vector<int2> ach_items;
     size_t tab_inst = 0, tab_goff = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
 };
