@@ -186,6 +186,14 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
     const double limitone = 1.0 / (kTo1 * 4.0);
+    // conj(Bg[n1][j]) of the occupied bands, loaded once for the column (the
+    // optional p2/br1 stores in the loop would otherwise keep the compiler
+    // from hoisting these loads out of it).
+    cplx bgc[NV > 0 ? NV : 1];
+    if constexpr (NV > 0) {
+#pragma unroll
+        for (int n1 = 0; n1 < NV; ++n1) bgc[n1] = cconj(a.bg[(size_t)n1 * a.ngpown + j]);
+    }
     // Frequencies in passes of FC; the passes after the first re-read this
     // block's G-vector range, recently fetched, from the caches.
     for (int iw0 = iw_begin; iw0 < iw_end; iw0 += FC) {
@@ -212,9 +220,14 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         cplx mo = czero(), om2 = czero(), mum = mu, qh = czero();
         double ie_cut = 0.0, wti2 = 0.0;
         if (nv > 0) {
+            if constexpr (NV > 0) {
 #pragma unroll
-            for (int n1 = 0; n1 < nv; ++n1)  // occupied bands, in band order
-                mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
+                for (int n1 = 0; n1 < NV; ++n1)  // occupied bands, in band order
+                    mo = cadd(mo, cmul_fast(bgc[n1], a.aqsn[(size_t)n1 * ncouls + ig]));
+            } else {
+                for (int n1 = 0; n1 < nv; ++n1)
+                    mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
+            }
             om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
             mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
             qh = cscale(cmul_fast(ie, wt), 0.5);
