@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restr
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
 // compile time (0: any).
 template <int FC, int NV>
-__global__ __launch_bounds__(kRedThreads) void k_pole_reduce(ReduceArgs a)
+__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 4 : 2) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
