@@ -232,7 +232,12 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
             qh = cscale(cmul_fast(ie, wt), 0.5);
             wti2 = wt.im * wt.im;
-            if (any_neg) ie_cut = cabs_(ie) * kSexcut;
+            if (any_neg) {
+                // A real (uniform) branch: the empty asm keeps the compiler
+                // from evaluating the square root for every element.
+                asm volatile("" ::: "memory");
+                ie_cut = cabs_(ie) * kSexcut;
+            }
         }
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
