@@ -21,6 +21,10 @@ SHAPES = [  # nb, nv, ncouls, ngpown, nfreq
     (17, 3, 513, 25, 5),
     (6, 6, 300, 12, 3),
     (33, 2, 1000, 50, 16),
+    (20, 8, 400, 20, 4),   # the reduce kernels are specialised per occupied count:
+    (15, 1, 350, 14, 3),   # 8, 5 and 7 here, 1 and 0 take the generic kernel
+    (21, 5, 450, 22, 7),
+    (19, 7, 380, 19, 2),
 ]
 
 
