@@ -118,19 +118,44 @@ def run_c4(args, cpu):
         gpu_ms.append(batch.last_timing()["ms"])
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     cfg = batch.last_timing()
+    # the same batch with the Ozaki INT8 and FP16 Gram products (autotuned reduce)
+    oz = {}
+    for form in ("factored-ozaki-i8", "factored-ozaki"):
+        batch.autotune(form)
+        batch.use_graph(False)
+        batch.sigma(form)
+        st = batch.last_timing()["stages_ms"]
+        batch.use_graph(True)
+        for _ in range(args.warmup):
+            batch.sigma(form)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            o = batch.sigma(form)
+        oms = (time.perf_counter() - t1) * 1e3 / args.steps
+        oz[form] = {"ms_per_step": oms, "updates_per_s": updates / (oms * 1e-3), "stages_ms_eager": st,
+                    "outs": o}
     # per-instance FP64 references (direct kernel) and the one-at-a-time factored path
     errs, single_ms, direct_ms = [], 0.0, 0.0
-    for (p, seed, wxc, dw), out in zip(inst, outs):
+    oz_errs = {f: [] for f in oz}
+    for i, ((p, seed, wxc, dw), out) in enumerate(zip(inst, outs)):
         ctx.generate_seeded(p, seed, wxc, dw)
         ref = ctx.sigma("direct")
         direct_ms += ctx.last_timing()["total"]
         ctx.sigma("factored-f64")
         single_ms += ctx.last_timing()["total"]
         errs.append({k: rel_err(out[k], ref[k]) for k in ("achtemp", "asxtemp", "achstemp")})
+        for f in oz:
+            oz_errs[f].append({k: rel_err(oz[f]["outs"][i][k], ref[k]) for k in ("achtemp", "asxtemp", "achstemp")})
+            oz_errs[f][-1].update({k + "_modulus": rel_err_modulus(oz[f]["outs"][i][k], ref[k])
+                                   for k in ("achtemp", "asxtemp", "achstemp")})
+    for f in oz:
+        del oz[f]["outs"]
+        oz[f]["max_rel_err_vs_direct_per_output"] = {k: max(e[k] for e in oz_errs[f]) for k in oz_errs[f][0]}
+        oz[f]["speedup_vs_direct"] = direct_ms / oz[f]["ms_per_step"]
     batch.close()
     ctx.close()
     gram_flop = sum(8 * p.ngpown * p.ncouls * (p.number_bands - p.nvband) for p in probs)
-    achieved = gram_flop / (stages["gram_f64"] * 1e-3) / 1e12
+    achieved = gram_flop / (stages["gram"] * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": updates / (ms * 1e-3), "unit": "updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -145,7 +170,7 @@ def run_c4(args, cpu):
         "roofline": {"bound": "mfma", "kernel": "k_batch_gram (v_mfma_f64_16x16x4, persistent)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "algorithmic_flop_per_launch": gram_flop,
-                     "avg_launch_ms": stages["gram_f64"], "stages_ms_eager": stages},
+                     "avg_launch_ms": stages["gram"], "stages_ms_eager": stages},
         "cpu_baseline": cpu,
         "batch": {"graph_gpu_ms_avg": sum(gpu_ms) / len(gpu_ms),
                   "max_rel_err_vs_direct_per_output": {k: max(e[k] for e in errs) for k in errs[0]},
@@ -153,7 +178,8 @@ def run_c4(args, cpu):
                   "one_at_a_time_factored_gpu_ms": single_ms,
                   "direct_gpu_ms": direct_ms,
                   "speedup_vs_direct": direct_ms / ms,
-                  "speedup_vs_one_at_a_time": single_ms / ms},
+                  "speedup_vs_one_at_a_time": single_ms / ms,
+                  "ozaki_forms": oz},
     }
     if cpu:
         line["gpu_over_cpu"] = line["value"] / cpu["value"]
@@ -166,6 +192,14 @@ def load_traffic(kernel):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+def rel_err_modulus(a, b):
+    """max |a - b| / |b| over the complex entries (normwise per output entry)."""
+    import numpy as np
+    a = np.atleast_1d(np.asarray(a, np.complex128))
+    b = np.atleast_1d(np.asarray(b, np.complex128))
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
 
 
 def rel_err(a, b):

@@ -976,6 +976,27 @@ namespace {
 
 int batch_blocks(const gpp_batch *b) { return 2 * b->ctx->num_cus; }
 
+// The batch's GramKind for `form`: its Gram tile and reduce schedule, with
+// the Gram product of the form (FP64 MFMA, or the Ozaki split with the
+// context's slice count).
+gpp_status batch_kind(const gpp_batch *b, gpp_form form, gpp::GramKind *k)
+{
+    *k = b->kind;
+    k->ozaki_slices = 0;
+    k->int8 = false;
+    if (form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) {
+        k->ozaki_slices = b->ctx->ozaki_slices;
+        k->int8 = form == GPP_FORM_FACTORED_OZAKI_I8;
+        for (int i = 0; i < b->n; ++i)
+            if (k->int8 && b->dps[i].nb - b->dps[i].nv > gpp::kOzakiInt8MaxK)
+                return fail(GPP_ERR_UNSUPPORTED, "instance %d: %d unoccupied bands exceed the INT8 form's %d", i,
+                            b->dps[i].nb - b->dps[i].nv, gpp::kOzakiInt8MaxK);
+    } else if (form != GPP_FORM_FACTORED_F64) {
+        return fail(GPP_ERR_UNSUPPORTED, "batches run the factored forms only");
+    }
+    return GPP_OK;
+}
+
 void batch_drop_graph(gpp_batch *b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -1139,7 +1160,9 @@ gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs
 gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
 {
     if (!b || !outs) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (form != GPP_FORM_FACTORED_F64) return fail(GPP_ERR_UNSUPPORTED, "batches run GPP_FORM_FACTORED_F64 only");
+    gpp::GramKind kind;
+    gpp_status stk = batch_kind(b, form, &kind);
+    if (stk) return stk;
     for (int i = 0; i < b->n; ++i) {
         if (!outs[i].achtemp || !outs[i].asxtemp)
             return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: achtemp and asxtemp are required", i);
@@ -1149,7 +1172,9 @@ gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
     gpp_status st = set_device(b->ctx);
     if (st) return st;
     hipStream_t s = b->ctx->stream;
-    if (!b->planned && (st = batch_prepare(b, b->kind))) return st;
+    if ((!b->planned || kind.ozaki_slices != b->kind.ozaki_slices || kind.int8 != b->kind.int8) &&
+        (st = batch_prepare(b, kind)))
+        return st;
     if (b->use_graph && !b->exec && (st = batch_capture(b))) return st;
     HIP_OR_FAIL(hipEventRecord(b->ev0, s), "hipEventRecord");
     if (b->use_graph) {
@@ -1193,8 +1218,10 @@ gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
 gpp_status gpp_batch_autotune(gpp_batch *b, gpp_form form)
 {
     if (!b) return fail(GPP_ERR_INVALID_ARGUMENT, "NULL batch");
-    if (form != GPP_FORM_FACTORED_F64) return fail(GPP_ERR_UNSUPPORTED, "batches run GPP_FORM_FACTORED_F64 only");
-    gpp_status st = set_device(b->ctx);
+    gpp::GramKind kind;
+    gpp_status st = batch_kind(b, form, &kind);
+    if (st) return st;
+    st = set_device(b->ctx);
     if (st) return st;
     hipStream_t s = b->ctx->stream;
     // time the stages selected by `which` for kind k (best of two after a warm-up)
@@ -1215,9 +1242,9 @@ gpp_status gpp_batch_autotune(gpp_batch *b, gpp_form form)
         *ms_out = best;
         return GPP_OK;
     };
-    gpp::GramKind k = b->kind, best_k = b->kind;
+    gpp::GramKind k = kind, best_k = kind;
     double best = 1e300, ms = 0;
-    for (int tile = 0; tile < gpp::kGramTiles; ++tile) {
+    for (int tile = 0; tile < (kind.ozaki_slices ? 0 : gpp::kGramTiles); ++tile) {  // FP64 Gram tiles only
         k.tile = tile;
         if ((st = timed(k, 2, &ms))) return st;
         if (ms < best) best = ms, best_k.tile = tile;

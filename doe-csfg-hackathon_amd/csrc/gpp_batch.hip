@@ -12,10 +12,15 @@
 // static round-robin (block b takes items b, b+G, b+2G, ...), so every item is
 // computed by exactly one block and results do not depend on scheduling.
 // The five launches are captured once into a hipGraph and replayed.
+//
+// The Ozaki kinds replace the Gram launch by three: column exponents and
+// digit planes of every instance's Bg and aqsn (gpp_ozaki_dev.hpp), then the
+// layout-3 INT8 / FP16 digit product over (instance, 64 x 64 tile) items.
 #include <algorithm>
 #include <vector>
 
 #include "gpp_factored_dev.hpp"
+#include "gpp_ozaki_dev.hpp"
 
 namespace gpp {
 namespace {
@@ -29,6 +34,10 @@ struct BInst {
     int nb, nv, ncouls, ngpown, nfreq, nsplit;
     double occ;
     cplx *bg, *mu, *col_ach, *col_asx, *part_achs, *out;
+    // Ozaki kinds: digit planes and column exponents of Bg (b) and aqsn (a)
+    void *oz_bd, *oz_ad;
+    int *oz_eb, *oz_ea;
+    int oz_ngp_pad, oz_nc_pad, oz_kpad;
 };
 
 __device__ __forceinline__ int find_inst(const size_t *off, int n, size_t t)
@@ -75,6 +84,60 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_batch_gram(const BInst *__re
         const GramItem it = items[w];
         const BInst &b = insts[it.inst];
         gram_f64_tile<WJN, WIN>(b.bg, b.aqsn, b.ngpown, b.ncouls, b.nv, b.nb, it.j0, it.i0, sB, sA, b.mu);
+    }
+}
+
+// Column exponents of item (inst, side, 64-column block): side 0 = Bg's
+// columns over bands [nv, nb), side 1 = aqsn's.
+__global__ __launch_bounds__(256) void k_batch_oz_colexp(const BInst *__restrict__ insts,
+                                                         const int4 *__restrict__ items, int nitems)
+{
+    __shared__ double part[4][64];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const int4 it = items[w];
+        const BInst &b = insts[it.x];
+        if (it.y == 0)
+            oz::colexp_block(b.bg, b.ngpown, b.nv, b.nb, b.ngpown, b.oz_ngp_pad, b.oz_eb, it.z, part);
+        else
+            oz::colexp_block(b.aqsn, b.ncouls, b.nv, b.nb, b.ncouls, b.oz_nc_pad, b.oz_ea, it.z, part);
+        __syncthreads();  // part is reused by the next item
+    }
+}
+
+// Digit pieces of item (inst, side, 256-column block, piece).
+template <int S, typename D>
+__global__ __launch_bounds__(256) void k_batch_oz_split(const BInst *__restrict__ insts,
+                                                        const int4 *__restrict__ items, int nitems)
+{
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const int4 it = items[w];
+        const BInst &b = insts[it.x];
+        const int K = b.nb - b.nv;
+        const int c = it.z * 256 + threadIdx.x;
+        if (it.y == 0) {
+            if (c < b.oz_ngp_pad)
+                oz::split_piece<S, D>(b.bg, b.ngpown, b.nv, K, b.ngpown, b.oz_ngp_pad, b.oz_kpad, b.oz_eb,
+                                      static_cast<D *>(b.oz_bd), c, it.w);
+        } else if (c < b.oz_nc_pad) {
+            oz::split_piece<S, D>(b.aqsn, b.ncouls, b.nv, K, b.ncouls, b.oz_nc_pad, b.oz_kpad, b.oz_ea,
+                                  static_cast<D *>(b.oz_ad), c, it.w);
+        }
+    }
+}
+
+// Layout-3 digit-product tiles, one (instance, 64 x 64 tile) item at a time.
+template <int S, typename D, int NBUF>
+__global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restrict__ insts,
+                                                          const GramItem *__restrict__ items, int nitems)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF][oz::Oz3<S>::STAGE];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const GramItem it = items[w];
+        const BInst &b = insts[it.inst];
+        oz::barrier_no_fence();  // the previous tile's last stage reads are done before its buffers refill
+        oz::gram_tile32<S, D, NBUF>(static_cast<const D *>(b.oz_bd), static_cast<const D *>(b.oz_ad), b.oz_eb,
+                                    b.oz_ea, b.ngpown, b.ncouls, b.oz_ngp_pad, b.oz_nc_pad, b.oz_kpad, it.j0,
+                                    it.i0, b.mu, lds);
     }
 }
 
@@ -141,6 +204,15 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         o.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
         o.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
         o.out = off;       off += align256(sizeof(cplx) * ((size_t)2 * p.nfreq + p.nb + 1));
+        o.oz_eb = o.oz_ea = o.oz_bd = o.oz_ad = 0;
+        if (k.ozaki_slices) {
+            const oz::OzLayout L = oz::oz_layout(p, k.ozaki_slices, k.int8);
+            o.oz_eb = off + L.eb;
+            o.oz_ea = off + L.ea;
+            o.oz_bd = off + L.bd;
+            o.oz_ad = off + L.ad;
+            off += align256(L.total);
+        }
         pl.inst_off[i] = o;
     }
     // device tables after the per-instance areas
@@ -177,9 +249,32 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
     }
     pl.red_class_start[kRedClasses] = (int)pl.red_items.size();
     pl.ach_items = a;
+    pl.oz_exp_items.clear();
+    pl.oz_split_items.clear();
+    pl.oz_gram_items.clear();
+    if (k.ozaki_slices) {
+        const int KP = k.int8 ? 16 : 8;  // bands per 16-byte digit piece
+        std::vector<std::pair<long, GramItem>> og;
+        for (int i = 0; i < pl.n; ++i) {
+            const oz::OzLayout L = oz::oz_layout(probs[i], k.ozaki_slices, k.int8);
+            for (int side = 0; side < 2; ++side) {
+                const int npad = side ? L.nc_pad : L.ngp_pad;
+                for (int cb = 0; cb < npad / 64; ++cb) pl.oz_exp_items.push_back(int4{i, side, cb, 0});
+                for (int cb = 0; cb < (npad + 255) / 256; ++cb)
+                    for (int kq = 0; kq < L.kpad / KP; ++kq) pl.oz_split_items.push_back(int4{i, side, cb, kq});
+            }
+            for (int i0 = 0; i0 < L.nc_pad; i0 += oz::kOz3Tile)
+                for (int j0 = 0; j0 < L.ngp_pad; j0 += oz::kOz3Tile) og.push_back({L.kpad, GramItem{i, j0, i0, 0}});
+        }
+        std::stable_sort(og.begin(), og.end(), by_cost);
+        for (auto &e : og) pl.oz_gram_items.push_back(e.second);
+    }
     pl.tab_gram = off; off += align256(sizeof(GramItem) * (pl.gram_items.size() + 1));
     pl.tab_red = off;  off += align256(sizeof(RedItem) * (pl.red_items.size() + 1));
     pl.tab_ach = off;  off += align256(sizeof(int2) * (pl.ach_items.size() + 1));
+    pl.tab_oz_exp = off;   off += align256(sizeof(int4) * (pl.oz_exp_items.size() + 1));
+    pl.tab_oz_split = off; off += align256(sizeof(int4) * (pl.oz_split_items.size() + 1));
+    pl.tab_oz_gram = off;  off += align256(sizeof(GramItem) * (pl.oz_gram_items.size() + 1));
     pl.total = off;
     pl.fc = k.red_fc == 4 ? 4 : 8;
     if (work_bytes) *work_bytes = pl.total;
@@ -205,6 +300,19 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
         b.col_asx = reinterpret_cast<cplx *>(w + o.col_asx);
         b.part_achs = reinterpret_cast<cplx *>(w + o.part_achs);
         b.out = reinterpret_cast<cplx *>(w + o.out);
+        b.oz_bd = b.oz_ad = nullptr;
+        b.oz_eb = b.oz_ea = nullptr;
+        b.oz_ngp_pad = b.oz_nc_pad = b.oz_kpad = 0;
+        if (pl.kind.ozaki_slices) {
+            const oz::OzLayout L = oz::oz_layout(p, pl.kind.ozaki_slices, pl.kind.int8);
+            b.oz_bd = w + o.oz_bd;
+            b.oz_ad = w + o.oz_ad;
+            b.oz_eb = reinterpret_cast<int *>(w + o.oz_eb);
+            b.oz_ea = reinterpret_cast<int *>(w + o.oz_ea);
+            b.oz_ngp_pad = L.ngp_pad;
+            b.oz_nc_pad = L.nc_pad;
+            b.oz_kpad = L.kpad;
+        }
         goff[i + 1] = goff[i] + (size_t)p.nb * p.ngpown;
     }
     hipError_t e;
@@ -223,7 +331,37 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
         (e = hipMemcpyAsync(w + pl.tab_ach, pl.ach_items.data(), sizeof(int2) * pl.ach_items.size(),
                             hipMemcpyHostToDevice, s)))
         return e;
+    if (!pl.oz_exp_items.empty() &&
+        ((e = hipMemcpyAsync(w + pl.tab_oz_exp, pl.oz_exp_items.data(), sizeof(int4) * pl.oz_exp_items.size(),
+                             hipMemcpyHostToDevice, s)) ||
+         (e = hipMemcpyAsync(w + pl.tab_oz_split, pl.oz_split_items.data(),
+                             sizeof(int4) * pl.oz_split_items.size(), hipMemcpyHostToDevice, s)) ||
+         (e = hipMemcpyAsync(w + pl.tab_oz_gram, pl.oz_gram_items.data(),
+                             sizeof(GramItem) * pl.oz_gram_items.size(), hipMemcpyHostToDevice, s))))
+        return e;
     return hipStreamSynchronize(s);
+}
+
+template <int S, typename D>
+void launch_batch_oz_s(int G, hipStream_t s, const BInst *insts, const int4 *si, int ns, const GramItem *gi, int ng)
+{
+    constexpr int NBUF = oz::oz3_buffers(S);
+    hipLaunchKernelGGL((k_batch_oz_split<S, D>), dim3(std::max(1, std::min(G, ns))), dim3(256), 0, s, insts, si, ns);
+    // one 256-thread block per CU: G is two blocks per CU
+    hipLaunchKernelGGL((k_batch_gram_oz<S, D, NBUF>), dim3(std::max(1, std::min((G + 1) / 2, ng))), dim3(256), 0, s,
+                       insts, gi, ng);
+}
+
+template <typename D>
+void launch_batch_oz(int S, int G, hipStream_t s, const BInst *insts, const int4 *si, int ns, const GramItem *gi,
+                     int ng)
+{
+    switch (S) {
+    case 3: launch_batch_oz_s<3, D>(G, s, insts, si, ns, gi, ng); break;
+    case 4: launch_batch_oz_s<4, D>(G, s, insts, si, ns, gi, ng); break;
+    case 5: launch_batch_oz_s<5, D>(G, s, insts, si, ns, gi, ng); break;
+    default: launch_batch_oz_s<6, D>(G, s, insts, si, ns, gi, ng); break;
+    }
 }
 
 template <int FC>
@@ -254,7 +392,18 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         hipLaunchKernelGGL(k_batch_achstemp, dim3(std::min<size_t>(G, pl.ach_items.size() + 1)), dim3(kRedThreads), 0,
                            s, insts, reinterpret_cast<const int2 *>(w + pl.tab_ach), (int)pl.ach_items.size());
     }
-    if (which & 2) {
+    if ((which & 2) && pl.kind.ozaki_slices) {
+        const int4 *ei = reinterpret_cast<const int4 *>(w + pl.tab_oz_exp);
+        const int4 *si = reinterpret_cast<const int4 *>(w + pl.tab_oz_split);
+        const GramItem *gi = reinterpret_cast<const GramItem *>(w + pl.tab_oz_gram);
+        const int ne = (int)pl.oz_exp_items.size(), ns = (int)pl.oz_split_items.size();
+        const int ng = (int)pl.oz_gram_items.size();
+        hipLaunchKernelGGL(k_batch_oz_colexp, dim3(std::max(1, std::min(G, ne))), dim3(256), 0, s, insts, ei, ne);
+        if (pl.kind.int8)
+            launch_batch_oz<int8_t>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng);
+        else
+            launch_batch_oz<_Float16>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng);
+    } else if (which & 2) {
         const GramItem *gi = reinterpret_cast<const GramItem *>(w + pl.tab_gram);
         const int ng = (int)pl.gram_items.size();
         const dim3 grid(std::max(1, std::min(G, ng)));

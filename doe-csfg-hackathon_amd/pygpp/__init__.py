@@ -559,4 +559,4 @@ class Batch:
         st = (ctypes.c_double * 4)()
         _check(load().gpp_batch_last_stages(self._h, st))
         return {"ms": ms.value, "gram_tile": t.value, "reduce_chunk": c.value, "reduce_freqs": f.value,
-                "stages_ms": {"gather_achstemp": st[0], "gram_f64": st[1], "pole_reduce": st[2], "final": st[3]}}
+                "stages_ms": {"gather_achstemp": st[0], "gram": st[1], "pole_reduce": st[2], "final": st[3]}}

@@ -1,7 +1,8 @@
-"""The batched factored form: many uneven problems through one persistent
+"""The batched factored forms: many uneven problems through one persistent
 launch per stage, replayed from a captured hipGraph.  Each instance must
 match the oracle (1e-10 per component), and -- for the same tile and reduce
-schedule -- be bitwise identical to the single-problem factored path."""
+schedule -- be bitwise identical to the single-problem factored path, with
+the FP64 Gram product and with the Ozaki INT8 / FP16 ones."""
 import numpy as np
 import pytest
 
@@ -26,6 +27,18 @@ SHAPES = [  # nb, nv, ncouls, ngpown, nfreq
     (21, 5, 450, 22, 7),
     (19, 7, 380, 19, 2),
 ]
+
+
+def assert_close_modulus(out, ref, tol, keys):
+    """|out - ref| / |ref| per complex entry: the Ozaki split bounds the error
+    relative to the magnitude of each output (a real or imaginary part much
+    smaller than the modulus, from cancellation, carries an absolute error of
+    order 1e-12 |ref|, not a relative one)."""
+    for k in keys:
+        g = np.atleast_1d(np.asarray(out[k], np.complex128))
+        r = np.atleast_1d(np.asarray(ref[k], np.complex128))
+        err = float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-300)))
+        assert err <= tol, f"{k}: relative error {err:.3e} > {tol:.1e}"
 
 
 def make_arrays(seed0=100):
@@ -69,6 +82,48 @@ def test_batch_bitwise_equals_single_path(ctx):
         assert cfg["gram_tile"] == ctx.get_option(pygpp.GPP_OPT_GRAM_TILE)
         for k in KEYS:
             np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("form", ["factored-ozaki-i8", "factored-ozaki"])
+def test_batch_ozaki_vs_oracle_and_single_path(ctx, oracle, form):
+    """The Ozaki forms in a batch: every instance within 1e-10 of the oracle
+    and bitwise equal to the single-problem Ozaki path (same digit product,
+    same reduce)."""
+    import pygpp
+    arrays = make_arrays(seed0=300)
+    b, dev = device_batch(ctx, arrays)
+    try:
+        outs = b.sigma(form)
+    finally:
+        b.close()
+    for (a, nv), out in zip(arrays, outs):
+        assert_close_modulus(out, oracle.sigma(a, nvband=nv, threads=8), TOL, KEYS)
+    probs = [pygpp.Problem(*s) for s in SHAPES]
+    seeds = [31 + i for i in range(len(probs))]
+    with pygpp.Batch(ctx, probs, seeds=seeds) as bs:
+        outs = bs.sigma(form)
+    for p, s, out in zip(probs, seeds, outs):
+        ctx.generate_seeded(p, s)
+        ref = ctx.sigma(form)
+        for k in KEYS:
+            np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{form} {k}")
+
+
+def test_batch_switches_forms(ctx):
+    """One batch evaluated in turn with the FP64, INT8 and FP16 Gram products
+    (each re-plans the workspace and recaptures the graph) and back."""
+    import pygpp
+    probs = [pygpp.Problem(*s) for s in SHAPES[:4]]
+    with pygpp.Batch(ctx, probs, seeds=[5, 6, 7, 8]) as b:
+        first = b.sigma("factored-f64")
+        i8 = b.sigma("factored-ozaki-i8")
+        f16 = b.sigma("factored-ozaki")
+        again = b.sigma("factored-f64")
+    for x, y, z, w in zip(first, i8, f16, again):
+        for k in KEYS:
+            np.testing.assert_array_equal(x[k], w[k])
+        assert_close_modulus(y, x, TOL, KEYS)
+        assert_close_modulus(z, x, TOL, KEYS)
 
 
 def test_graph_replays_match_eager(ctx):
