@@ -393,11 +393,16 @@ gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs
                                    const double *wx_center, const double *dw, gpp_batch **out);
 
 /* Evaluates every instance; outs[i] receives instance i (acht_n1_loc must be
- * NULL).  Only GPP_FORM_FACTORED_F64 is batched. */
+ * NULL).  The factored forms are batched: GPP_FORM_FACTORED_F64,
+ * GPP_FORM_FACTORED_OZAKI and GPP_FORM_FACTORED_OZAKI_I8 (with the context's
+ * GPP_OPT_OZAKI_SLICES; the INT8 form needs nb - nvband <= 43689 in every
+ * instance).  A form other than the last one re-plans the workspace and
+ * re-captures the graph; results are bitwise equal to the single-problem
+ * path of the same form. */
 gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs);
 
-/* Times the Gram tiles and reduce schedules on this batch and keeps the
- * fastest (re-planning and re-capturing the graph). */
+/* Times the Gram tiles (FP64 form) and reduce schedules of `form` on this
+ * batch and keeps the fastest (re-planning and re-capturing the graph). */
 gpp_status gpp_batch_autotune(gpp_batch *b, gpp_form form);
 
 /* 1 (default): replay the captured hipGraph; 0: launch the same kernels eagerly. */
