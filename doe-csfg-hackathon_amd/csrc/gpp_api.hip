@@ -987,10 +987,6 @@ gpp_status batch_kind(const gpp_batch *b, gpp_form form, gpp::GramKind *k)
     if (form == GPP_FORM_FACTORED_OZAKI || form == GPP_FORM_FACTORED_OZAKI_I8) {
         k->ozaki_slices = b->ctx->ozaki_slices;
         k->int8 = form == GPP_FORM_FACTORED_OZAKI_I8;
-        for (int i = 0; i < b->n; ++i)
-            if (k->int8 && b->dps[i].nb - b->dps[i].nv > gpp::kOzakiInt8MaxK)
-                return fail(GPP_ERR_UNSUPPORTED, "instance %d: %d unoccupied bands exceed the INT8 form's %d", i,
-                            b->dps[i].nb - b->dps[i].nv, gpp::kOzakiInt8MaxK);
     } else if (form != GPP_FORM_FACTORED_F64) {
         return fail(GPP_ERR_UNSUPPORTED, "batches run the factored forms only");
     }

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_batch_oz_split(const BInst *__restrict_
 }
 
 // Layout-3 digit-product tiles, one (instance, 64 x 64 tile) item at a time.
-template <int S, typename D, int NBUF>
+template <int S, typename D, int NBUF, bool LONG>
 __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restrict__ insts,
                                                           const GramItem *__restrict__ items, int nitems)
 {
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
         const GramItem it = items[w];
         const BInst &b = insts[it.inst];
         oz::barrier_no_fence();  // the previous tile's last stage reads are done before its buffers refill
-        oz::gram_tile32<S, D, NBUF>(static_cast<const D *>(b.oz_bd), static_cast<const D *>(b.oz_ad), b.oz_eb,
+        oz::gram_tile32<S, D, NBUF, LONG>(static_cast<const D *>(b.oz_bd), static_cast<const D *>(b.oz_ad), b.oz_eb,
                                     b.oz_ea, b.ngpown, b.ncouls, b.oz_ngp_pad, b.oz_nc_pad, b.oz_kpad, it.j0,
                                     it.i0, b.mu, lds);
     }
@@ -252,6 +252,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
     pl.oz_exp_items.clear();
     pl.oz_split_items.clear();
     pl.oz_gram_items.clear();
+    pl.oz_long = false;
     if (k.ozaki_slices) {
         const int KP = k.int8 ? 16 : 8;  // bands per 16-byte digit piece
         std::vector<std::pair<long, GramItem>> og;
@@ -263,6 +264,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
                 for (int cb = 0; cb < (npad + 255) / 256; ++cb)
                     for (int kq = 0; kq < L.kpad / KP; ++kq) pl.oz_split_items.push_back(int4{i, side, cb, kq});
             }
+            if (k.int8 && L.kpad > oz::OzDigit<int8_t>::CHUNK) pl.oz_long = true;
             for (int i0 = 0; i0 < L.nc_pad; i0 += oz::kOz3Tile)
                 for (int j0 = 0; j0 < L.ngp_pad; j0 += oz::kOz3Tile) og.push_back({L.kpad, GramItem{i, j0, i0, 0}});
         }
@@ -343,24 +345,30 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
 }
 
 template <int S, typename D>
-void launch_batch_oz_s(int G, hipStream_t s, const BInst *insts, const int4 *si, int ns, const GramItem *gi, int ng)
+void launch_batch_oz_s(int G, hipStream_t s, const BInst *insts, const int4 *si, int ns, const GramItem *gi, int ng,
+                       bool long_range)
 {
     constexpr int NBUF = oz::oz3_buffers(S);
     hipLaunchKernelGGL((k_batch_oz_split<S, D>), dim3(std::max(1, std::min(G, ns))), dim3(256), 0, s, insts, si, ns);
     // one 256-thread block per CU: G is two blocks per CU
-    hipLaunchKernelGGL((k_batch_gram_oz<S, D, NBUF>), dim3(std::max(1, std::min((G + 1) / 2, ng))), dim3(256), 0, s,
-                       insts, gi, ng);
+    const dim3 grid(std::max(1, std::min((G + 1) / 2, ng)));
+    if (long_range)
+        hipLaunchKernelGGL((k_batch_gram_oz<S, D, NBUF, true>), grid, dim3(256), 0, s, insts, gi, ng);
+    else
+        hipLaunchKernelGGL((k_batch_gram_oz<S, D, NBUF, false>), grid, dim3(256), 0, s, insts, gi, ng);
 }
 
+// long_range: some instance's band range exceeds one exact INT8 chunk, so the
+// tiles flush per chunk (identical results for the shorter ranges).
 template <typename D>
 void launch_batch_oz(int S, int G, hipStream_t s, const BInst *insts, const int4 *si, int ns, const GramItem *gi,
-                     int ng)
+                     int ng, bool long_range)
 {
     switch (S) {
-    case 3: launch_batch_oz_s<3, D>(G, s, insts, si, ns, gi, ng); break;
-    case 4: launch_batch_oz_s<4, D>(G, s, insts, si, ns, gi, ng); break;
-    case 5: launch_batch_oz_s<5, D>(G, s, insts, si, ns, gi, ng); break;
-    default: launch_batch_oz_s<6, D>(G, s, insts, si, ns, gi, ng); break;
+    case 3: launch_batch_oz_s<3, D>(G, s, insts, si, ns, gi, ng, long_range); break;
+    case 4: launch_batch_oz_s<4, D>(G, s, insts, si, ns, gi, ng, long_range); break;
+    case 5: launch_batch_oz_s<5, D>(G, s, insts, si, ns, gi, ng, long_range); break;
+    default: launch_batch_oz_s<6, D>(G, s, insts, si, ns, gi, ng, long_range); break;
     }
 }
 
@@ -400,9 +408,9 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         const int ng = (int)pl.oz_gram_items.size();
         hipLaunchKernelGGL(k_batch_oz_colexp, dim3(std::max(1, std::min(G, ne))), dim3(256), 0, s, insts, ei, ne);
         if (pl.kind.int8)
-            launch_batch_oz<int8_t>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng);
+            launch_batch_oz<int8_t>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng, pl.oz_long);
         else
-            launch_batch_oz<_Float16>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng);
+            launch_batch_oz<_Float16>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng, false);
     } else if (which & 2) {
         const GramItem *gi = reinterpret_cast<const GramItem *>(w + pl.tab_gram);
         const int ng = (int)pl.gram_items.size();

@@ -170,13 +170,13 @@ struct BatchPlan {
     // pieces (inst, side, 256-column block, piece) and 64 x 64 Gram tiles.
     std::vector<int4> oz_exp_items, oz_split_items;
     std::vector<GramItem> oz_gram_items;
+    bool oz_long = false;  // an INT8 instance's band range exceeds one exact chunk
     size_t tab_inst = 0, tab_goff = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
     size_t tab_oz_exp = 0, tab_oz_split = 0, tab_oz_gram = 0;
 };
 
 BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes);
-// Largest band count (nb - nv) the INT8 Ozaki product sums exactly in INT32.
-constexpr int kOzakiInt8MaxK = 43689;
+
 // Writes the device tables of the plan into `work` (synchronous on s).
 hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &probs, void *work, hipStream_t s);
 // Enqueues the batch's persistent stages; which: bit 0 gather+achstemp, bit 1

@@ -23,10 +23,11 @@
 // converted to FP64 and accumulated.
 //
 // The same kernel runs with INT8 digits (v_mfma_i32_16x16x64_i8): digits in
-// [-64, 64] are exact int8, products accumulate exactly in INT32 while
-// 6 * 2 * 4096 * K < 2^31 (K < 43690 bands), so the levels are folded to FP64
-// once, after the whole band range; MI355X
-// runs INT8 MFMA at twice the FP16 rate and the staged digits are half the bytes.
+// [-64, 64] are exact int8, products accumulate exactly in INT32 over chunks
+// of 32768 bands (6 * 2 * 4096 * 32768 < 2^31), so up to that band count the
+// levels are folded to FP64 once, after the whole band range, and longer
+// ranges are flushed per chunk; MI355X runs INT8 MFMA at twice the FP16 rate
+// and the staged digits are half the bytes.
 //
 // MFMA lane layouts (measured on gfx950), 16 bytes of operand per lane:
 //   f32_16x16x32_f16: A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]
@@ -335,7 +336,7 @@ void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad, const int 
     }
 }
 
-template <int S, typename D, int NBUF>
+template <int S, typename D, int NBUF, bool LONG>
 __global__ __launch_bounds__(256, 1) void k_gram_ozaki32(const D *__restrict__ bd, const D *__restrict__ ad,
                                                          const int *__restrict__ eb, const int *__restrict__ ea,
                                                          int ngpown, int ncouls, int ngp_pad, int nc_pad, int kpad,
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(256, 1) void k_gram_ozaki32(const D *__restrict__ b
     const int t = xcd_tile(blockIdx.x, ntj * nti);
     const int g = t / (gj * nti), rem = t - g * gj * nti;
     const int gsz = min(gj, ntj - g * gj);
-    gram_tile32<S, D, NBUF>(bd, ad, eb, ea, ngpown, ncouls, ngp_pad, nc_pad, kpad, (g * gj + rem % gsz) * kOz3Tile,
+    gram_tile32<S, D, NBUF, LONG>(bd, ad, eb, ea, ngpown, ncouls, ngp_pad, nc_pad, kpad, (g * gj + rem % gsz) * kOz3Tile,
                             (rem / gsz) * kOz3Tile, mout, lds);
 }
 
@@ -384,15 +385,23 @@ hipError_t launch_ozaki_s(const DevProblem &p, int layout, int group, const cplx
                        p.ncouls, p.nv, L.K, p.ncouls, L.nc_pad, L.kpad, ea, ad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    switch (layout) {
+    // INT8 band ranges beyond one exact chunk run the layout-3 tile that
+    // flushes per chunk, whatever the layout option.
+    const bool long_range = OzDigit<D>::ONE_CHUNK && L.kpad > OzDigit<D>::CHUNK;
+    switch (long_range ? 3 : layout) {
     case 0: launch_gram_tiles<S, D, 0>(L, p, bd, ad, eb, ea, group, mout, s); break;
     case 1: launch_gram_tiles<S, D, 1>(L, p, bd, ad, eb, ea, group, mout, s); break;
     case 2: launch_gram_tiles<S, D, 2>(L, p, bd, ad, eb, ea, group, mout, s); break;
     default: {
         const int ntj = L.ngp_pad / kOz3Tile;
-        hipLaunchKernelGGL((k_gram_ozaki32<S, D, oz3_buffers(S)>), dim3((unsigned)ntj * (unsigned)(L.nc_pad / kOz3Tile)), dim3(256), 0,
-                           s, bd, ad, eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad,
-                           tile_group(group, ntj, (size_t)S * 2 * kOz3Tile * L.kpad * sizeof(D)), mout);
+        const dim3 grid((unsigned)ntj * (unsigned)(L.nc_pad / kOz3Tile));
+        const int gj = tile_group(group, ntj, (size_t)S * 2 * kOz3Tile * L.kpad * sizeof(D));
+        if (long_range)
+            hipLaunchKernelGGL((k_gram_ozaki32<S, D, oz3_buffers(S), true>), grid, dim3(256), 0, s, bd, ad, eb, ea,
+                               p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, gj, mout);
+        else
+            hipLaunchKernelGGL((k_gram_ozaki32<S, D, oz3_buffers(S), false>), grid, dim3(256), 0, s, bd, ad, eb, ea,
+                               p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad, gj, mout);
         break;
     }
     }
@@ -419,7 +428,6 @@ size_t ozaki_workspace_bytes(const DevProblem &p, int slices, bool int8) { retur
 hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int layout, int group, const cplx *bg,
                              void *work, cplx *mout, hipStream_t s)
 {
-    if (int8 && (p.nb - p.nv) >= 43690) return hipErrorInvalidValue;  // INT32 exactness bound
     return int8 ? launch_ozaki_d<int8_t>(p, slices, layout, group, bg, work, mout, s)
                 : launch_ozaki_d<_Float16>(p, slices, layout, group, bg, work, mout, s);
 }

@@ -50,7 +50,11 @@ template <> struct OzDigit<int8_t> {
     typedef v4i frag;
     typedef v4i acc;
     static constexpr int KS = 64;
-    static constexpr int CHUNK = 1 << 30;    // one chunk: exact over every admissible band count (K < 43690)
+    // INT32 sums are exact over 32768 bands (6 * 2 * 4096 * 32768 < 2^31):
+    // band ranges up to that are one chunk, folded to FP64 once at the end;
+    // longer ones run the tile instantiated for long ranges (LONG), which
+    // flushes to FP64 every 32768 bands like the FP16 chunks.
+    static constexpr int CHUNK = 32768;
     static constexpr bool ONE_CHUNK = true;
     static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
     {
@@ -195,7 +199,7 @@ template <int S> struct Oz3 {
     static constexpr int SIDE = S * PLANE;
     static constexpr int STAGE = 2 * SIDE;
 };
-template <int S, typename D, int NBUF>
+template <int S, typename D, int NBUF, bool LONG = false>
 __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *__restrict__ ad,
                                             const int *__restrict__ eb, const int *__restrict__ ea, int ngpown,
                                             int ncouls, int ngp_pad, int nc_pad, int kpad, int j0, int i0,
@@ -306,9 +310,10 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
 
     const int nst = kpad / KS;
     for (int st = 0; st < NBUF - 1 && st < nst; ++st) issue_stage(st, 0, PER_T);
-    // Chunks of stages (FP16: CHUNK bands; INT8: all) with the flush between
+    // Chunks of stages (CHUNK bands) with the flush between
     // them, outside the stage loop, so that the accumulators stay in place.
-    constexpr int CST = T::CHUNK / KS;
+    constexpr bool CHUNKED = !T::ONE_CHUNK || LONG;
+    constexpr int CST = CHUNKED ? T::CHUNK / KS : 1 << 30;
     bool drained = false;  // a flush waited for every outstanding load
     for (int c0 = 0; c0 < nst; c0 += CST) {
     const int c1 = min(nst, c0 + CST);
@@ -355,13 +360,13 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
             }
         }
     }
-    if constexpr (!T::ONE_CHUNK) {
+    if constexpr (CHUNKED) {
         flush(c0 == 0, c1 == nst);
         drained = true;
     }
     }
 
-    if constexpr (T::ONE_CHUNK) {
+    if constexpr (!CHUNKED) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int j = j0 + wj + res_row(r);

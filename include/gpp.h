@@ -91,8 +91,8 @@ typedef enum {
      * FP64; slices per operand from GPP_OPT_OZAKI_SLICES. */
     GPP_FORM_FACTORED_OZAKI = 3,
     /* As GPP_FORM_FACTORED_OZAKI with the same digits held in INT8 and
-     * multiplied on INT8 MFMA with exact INT32 accumulation (needs
-     * number_bands - nvband < 43690). */
+     * multiplied on INT8 MFMA with exact INT32 accumulation (flushed to FP64
+     * every 32768 bands). */
     GPP_FORM_FACTORED_OZAKI_I8 = 4
 } gpp_form;
 
@@ -395,8 +395,7 @@ gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs
 /* Evaluates every instance; outs[i] receives instance i (acht_n1_loc must be
  * NULL).  The factored forms are batched: GPP_FORM_FACTORED_F64,
  * GPP_FORM_FACTORED_OZAKI and GPP_FORM_FACTORED_OZAKI_I8 (with the context's
- * GPP_OPT_OZAKI_SLICES; the INT8 form needs nb - nvband <= 43689 in every
- * instance).  A form other than the last one re-plans the workspace and
+ * GPP_OPT_OZAKI_SLICES).  A form other than the last one re-plans the workspace and
  * re-captures the graph; results are bitwise equal to the single-problem
  * path of the same form. */
 gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs);

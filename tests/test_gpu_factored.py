@@ -158,6 +158,28 @@ def test_ozaki_layouts_and_digit_types_agree(ctx, oracle, nb):
     assert_close(out[("factored-ozaki-i8", 2, 0)], ref, TOL_FACTORED, KEYS)
 
 
+def test_ozaki_long_band_range(ctx, oracle):
+    """33000 unoccupied bands: the INT8 form flushes its exact INT32 level
+    sums to FP64 after 32768 bands (two chunks), the FP16 form after every 256;
+    both within 1e-10 of the oracle and of each other's exact-sum results."""
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(33002, 96, 6, nfreq=3, seed=41)
+    ctx.upload(problem_of(arrs, 2), arrs)
+    ref = oracle.sigma(arrs, nvband=2, threads=8)
+    out = {}
+    for lay in (3, 1):
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, lay)
+        try:
+            for form in ("factored-ozaki-i8", "factored-ozaki"):
+                out[(form, lay)] = ctx.sigma(form)
+        finally:
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 3)
+    for form in ("factored-ozaki-i8", "factored-ozaki"):
+        assert_close(out[(form, 3)], ref, TOL_FACTORED, KEYS)
+        for k in KEYS:
+            np.testing.assert_array_equal(out[(form, 1)][k], out[(form, 3)][k])
+
+
 def test_ozaki_option_validation(ctx):
     import pygpp
     for bad in (2, 7):
