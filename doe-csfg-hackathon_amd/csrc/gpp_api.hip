@@ -183,7 +183,7 @@ struct gpp_ctx {
     gpp::StageTimer timer;
     bool timed = false;
     int ozaki_slices = 6;
-    int ozaki_layout = 3;
+    int ozaki_layout = 4;
     int ozaki_group = 0;
     int gram_tile = 0;
     int red_chunk = 8192;
@@ -766,7 +766,7 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         ctx->ozaki_slices = value;
         return GPP_OK;
     case GPP_OPT_OZAKI_LAYOUT:
-        if (value < 0 || value > 3) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0..3, got %d", value);
+        if (value < 0 || value > 4) return fail(GPP_ERR_INVALID_ARGUMENT, "ozaki layout must be 0..4, got %d", value);
         ctx->ozaki_layout = value;
         return GPP_OK;
     case GPP_OPT_OZAKI_GROUP:

@@ -125,20 +125,32 @@ __global__ __launch_bounds__(256) void k_batch_oz_split(const BInst *__restrict_
     }
 }
 
-// Layout-3 digit-product tiles, one (instance, 64 x 64 tile) item at a time.
+// The (instance, 64 x 64 tile) items of a persistent block, items b, b + G,
+// b + 2G, ... (G = grid size).
+struct OzItemSeq {
+    const BInst *insts;
+    const GramItem *items;
+    int nitems;
+    __device__ bool get(int k, oz::OzTile &t) const
+    {
+        const long long w = (long long)blockIdx.x + (long long)k * gridDim.x;
+        if (w >= nitems) return false;
+        const GramItem it = items[w];
+        const BInst &b = insts[it.inst];
+        t = oz::OzTile{b.oz_bd, b.oz_ad, b.oz_eb, b.oz_ea, b.ngpown, b.ncouls, b.oz_ngp_pad, b.oz_nc_pad,
+                       b.oz_kpad, it.j0, it.i0, b.mu};
+        return true;
+    }
+};
+
+// Layout-3 digit-product tiles over all instances as one stream of stages per
+// persistent block (the next item's first stages load while one finishes).
 template <int S, typename D, int NBUF, bool LONG>
 __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restrict__ insts,
                                                           const GramItem *__restrict__ items, int nitems)
 {
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF][oz::Oz3<S>::STAGE];
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const GramItem it = items[w];
-        const BInst &b = insts[it.inst];
-        oz::barrier_no_fence();  // the previous tile's last stage reads are done before its buffers refill
-        oz::gram_tile32<S, D, NBUF, LONG>(static_cast<const D *>(b.oz_bd), static_cast<const D *>(b.oz_ad), b.oz_eb,
-                                    b.oz_ea, b.ngpown, b.ncouls, b.oz_ngp_pad, b.oz_nc_pad, b.oz_kpad, it.j0,
-                                    it.i0, b.mu, lds);
-    }
+    oz::gram_tiles32<S, D, NBUF, LONG>(OzItemSeq{insts, items, nitems}, lds);
 }
 
 // Reduce items of instances whose occupied band count is NV (0: any count).

@@ -106,7 +106,8 @@ hipError_t launch_fill_seeded(cplx *dst, size_t n, uint64_t seed, uint32_t array
 struct GramKind {
     int ozaki_slices = 0;  // 0: FP64 MFMA; 3..6: Ozaki split with that many slices
     bool int8 = false;     // Ozaki digits in INT8 (else FP16)
-    int layout = 3;        // Ozaki block layout (0, 1: 64x64, registers staged; 2: 64x32, 3: 64x64 32x32 MFMA, direct loads)
+    int layout = 4;        // Ozaki block layout (0, 1: 64x64 staged in registers; 2: 64x32; 3: 64x64 on 32x32 MFMAs;
+                           // 4: layout 3 in persistent blocks; 2-4 load straight into LDS)
     int group = 0;         // Ozaki G' tiles per tile group (0: sized for the L2)
     int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
     int red_chunk = 8192;  // G-vectors per pole-reduce block

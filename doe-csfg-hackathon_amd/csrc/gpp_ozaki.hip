@@ -351,6 +351,50 @@ __global__ __launch_bounds__(256, 1) void k_gram_ozaki32(const D *__restrict__ b
                             (rem / gsz) * kOz3Tile, mout, lds);
 }
 
+// Layout 4: the layout-3 tiles in persistent blocks (one per CU), the block's
+// tiles taken in the same grouped, XCD-ordered sequence with a stride of the
+// grid, as one stream of stages (the next tile's first stages load while the
+// current one finishes).
+template <typename D> struct OzTileSeq {
+    const D *bd, *ad;
+    const int *eb, *ea;
+    int ngpown, ncouls, ngp_pad, nc_pad, kpad, gj;
+    cplx *mout;
+    __device__ bool get(int k, OzTile &t) const
+    {
+        const int ntj = ngp_pad / kOz3Tile, nti = nc_pad / kOz3Tile, nt = ntj * nti;
+        const long long b = (long long)blockIdx.x + (long long)k * gridDim.x;
+        if (b >= nt) return false;
+        const int tt = xcd_tile((int)b, nt);
+        const int g = tt / (gj * nti), rem = tt - g * gj * nti;
+        const int gsz = min(gj, ntj - g * gj);
+        t = OzTile{bd, ad, eb, ea, ngpown, ncouls, ngp_pad, nc_pad, kpad, (g * gj + rem % gsz) * kOz3Tile,
+                   (rem / gsz) * kOz3Tile, mout};
+        return true;
+    }
+};
+
+template <int S, typename D, int NBUF, bool LONG>
+__global__ __launch_bounds__(256, 1) void k_gram_ozaki32p(OzTileSeq<D> seq)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF][Oz3<S>::STAGE];
+    gram_tiles32<S, D, NBUF, LONG>(seq, lds);
+}
+
+// Compute units of the current device (cached per device).
+int device_cus()
+{
+    static int cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
+
 template <int S, typename D, int LAYOUT>
 void launch_gram_tiles(const OzLayout &L, const DevProblem &p, const D *bd, const D *ad, const int *eb, const int *ea,
                        int group, cplx *mout, hipStream_t s)
@@ -388,6 +432,19 @@ hipError_t launch_ozaki_s(const DevProblem &p, int layout, int group, const cplx
     // INT8 band ranges beyond one exact chunk run the layout-3 tile that
     // flushes per chunk, whatever the layout option.
     const bool long_range = OzDigit<D>::ONE_CHUNK && L.kpad > OzDigit<D>::CHUNK;
+    if (layout == 4) {  // persistent layout 3
+        const int ntj = L.ngp_pad / kOz3Tile;
+        const unsigned nt = (unsigned)ntj * (unsigned)(L.nc_pad / kOz3Tile);
+        const unsigned cus = (unsigned)device_cus();
+        const OzTileSeq<D> seq{bd, ad, eb, ea, p.ngpown, p.ncouls, L.ngp_pad, L.nc_pad, L.kpad,
+                               tile_group(group, ntj, (size_t)S * 2 * kOz3Tile * L.kpad * sizeof(D)), mout};
+        const dim3 grid(nt < cus ? nt : cus);
+        if (long_range)
+            hipLaunchKernelGGL((k_gram_ozaki32p<S, D, oz3_buffers(S), true>), grid, dim3(256), 0, s, seq);
+        else
+            hipLaunchKernelGGL((k_gram_ozaki32p<S, D, oz3_buffers(S), false>), grid, dim3(256), 0, s, seq);
+        return hipGetLastError();
+    }
     switch (long_range ? 3 : layout) {
     case 0: launch_gram_tiles<S, D, 0>(L, p, bd, ad, eb, ea, group, mout, s); break;
     case 1: launch_gram_tiles<S, D, 1>(L, p, bd, ad, eb, ea, group, mout, s); break;

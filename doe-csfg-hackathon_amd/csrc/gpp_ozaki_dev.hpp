@@ -278,9 +278,18 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
             sm = sm * 0x1p-7 + (double)acc_im[t][r];
         }
     };
+    // Column exponents of this lane's 16 results (eb, ea hold the padded
+    // columns, so the clamped loads are all in bounds and issue together).
+    auto load_exps = [&](int (&sc)[16]) {
+        const int ea_i = ea[min(ig, nc_pad - 1)];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) sc[r] = eb[min(j0 + wj + res_row(r), ngp_pad - 1)] + ea_i - 12;
+    };
     // Chunked exact sums (FP16): running FP64 sums of the chunks kept,
     // unscaled, in the thread's own output elements of mout.
     auto flush = [&](bool first, bool last) {
+        int sc[16];
+        if (last) load_exps(sc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int j = j0 + wj + res_row(r);
@@ -294,9 +303,8 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
                     sm = prev.im + sm;
                 }
                 if (last) {
-                    const int sc = eb[j] + ea[ig] - 12;
-                    sr = ldexp(sr, sc);
-                    sm = ldexp(sm, sc);
+                    sr = ldexp(sr, sc[r]);
+                    sm = ldexp(sm, sc[r]);
                 }
                 *o = cplx{sr, sm};
             }
@@ -367,16 +375,236 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
     }
 
     if constexpr (!CHUNKED) {
+        int sc[16];
+        load_exps(sc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int j = j0 + wj + res_row(r);
             if (j < ngpown && ig < ncouls) {
-                const int sc = eb[j] + ea[ig] - 12;
                 double sr, sm;
                 fold(r, sr, sm);
-                mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc), ldexp(sm, sc)};
+                mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc[r]), ldexp(sm, sc[r])};
             }
         }
+    }
+}
+
+// One tile of the layout-3 product as handed to gram_tiles32.
+struct OzTile {
+    const void *bd, *ad;
+    const int *eb, *ea;
+    int ngpown, ncouls, ngp_pad, nc_pad, kpad, j0, i0;
+    cplx *mout;
+};
+
+// The tiles src.get(0), src.get(1), ... of a persistent block (get returns
+// false past the last) as ONE stream of stages: stage g of the stream uses
+// buffer g % NBUF and its loads are issued NBUF - 1 stages ahead, across tile
+// boundaries, so a tile's first stages load while the previous tile
+// finishes; otherwise as gram_tile32.
+template <int S, typename D, int NBUF, bool LONG, class Src>
+__device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds)[Oz3<S>::STAGE])
+{
+    typedef OzDigit<D> T;
+    typedef typename T::frag frag;
+    typedef typename T::acc32 acc_t;
+    constexpr int KS = T::KS32;
+    constexpr int KP = 16 / (int)sizeof(D);
+    constexpr int PLANE = 64 * 64;
+    constexpr int SIDE = S * PLANE;
+    constexpr int UB = S * 64 * 4;
+    constexpr int PER_T = 2 * UB / 256;
+    static_assert(PER_T * 256 == 2 * UB && UB % kWave == 0 && PER_T == 2 * S,
+                  "stage must split evenly, waves within a side, two units per digit round");
+    static_assert(NBUF >= 2 && NBUF <= 4 && NBUF * 2 * SIDE <= kLdsBytes, "two to four stage buffers in LDS");
+    constexpr bool CHUNKED = !T::ONE_CHUNK || LONG;
+    constexpr int CST = CHUNKED ? T::CHUNK / KS : 1 << 30;
+
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int wj = (wave / 2) * 32, wi = (wave % 2) * 32;
+    int fo_b[2], fo_a[2];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        const int h = lane / 32;
+        const int rb = wj + lane % 32, ra = wi + lane % 32;
+        fo_b[part] = rb * 64 + 16 * ((2 * part + h) ^ ((rb >> 2) & 3));
+        fo_a[part] = ra * 64 + 16 * ((2 * part + h) ^ ((ra >> 2) & 3));
+    }
+    const int urow = threadIdx.x / 4, uunit = (threadIdx.x % 4) ^ ((urow >> 2) & 3);
+
+    // Issue side: stage pst of tile pk (descriptor pt) is global stage gi.
+    OzTile pt;
+    int pk = 0, pst = 0, gi = 0, pnst = 0;
+    bool pvalid = src.get(0, pt);
+    if (!pvalid) return;
+    const unsigned char *pgb = nullptr, *pga = nullptr;
+    size_t poff_b = 0, poff_a = 0, pplane_b = 0, pplane_a = 0;
+    auto set_issue_tile = [&]() {
+        const int npieces = pt.kpad / KP;
+        pnst = pt.kpad / KS;
+        pgb = static_cast<const unsigned char *>(pt.bd);
+        pga = static_cast<const unsigned char *>(pt.ad);
+        poff_b = (((size_t)(uunit / 2) * npieces + uunit % 2) * pt.ngp_pad + pt.j0 + urow) * 16;
+        poff_a = (((size_t)(uunit / 2) * npieces + uunit % 2) * pt.nc_pad + pt.i0 + urow) * 16;
+        pplane_b = (size_t)2 * npieces * pt.ngp_pad * 16;
+        pplane_a = (size_t)2 * npieces * pt.nc_pad * 16;
+    };
+    set_issue_tile();
+    auto issue_units = [&](int q0, int q1) {
+        unsigned char *buf = lds[gi % NBUF] + 16 * (threadIdx.x & ~(kWave - 1));
+        const size_t kb = (size_t)pst * (KS / KP) * 16;
+#pragma unroll
+        for (int q = q0; q < q1; ++q) {
+            const unsigned char *s = q < S ? pgb + (kb * pt.ngp_pad + q * pplane_b + poff_b)
+                                           : pga + (kb * pt.nc_pad + (q - S) * pplane_a + poff_a);
+            load_lds16(s, buf + 4096 * q);
+        }
+    };
+    auto advance_issue = [&]() {
+        ++gi;
+        if (++pst == pnst) {
+            pst = 0;
+            ++pk;
+            pvalid = src.get(pk, pt);
+            if (pvalid) set_issue_tile();
+        }
+    };
+    for (int i = 0; i < NBUF - 1 && pvalid; ++i) {
+        issue_units(0, PER_T);
+        advance_issue();
+    }
+
+    // Compute side: tile ck (descriptor ct), global stage g.
+    OzTile ct;
+    (void)src.get(0, ct);
+    int ck = 0, g = 0;
+    acc_t acc_re[S], acc_im[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        acc_re[t] = acc_t{};
+        acc_im[t] = acc_t{};
+    }
+    auto res_row = [&](int r) { return 8 * (r / 4) + 4 * (lane / 32) + r % 4; };
+    auto fold = [&](int r, double &sr, double &sm) {
+        sr = 0.0;
+        sm = 0.0;
+#pragma unroll
+        for (int t = S - 1; t >= 0; --t) {
+            sr = sr * 0x1p-7 + (double)acc_re[t][r];
+            sm = sm * 0x1p-7 + (double)acc_im[t][r];
+        }
+    };
+    // first: the tile's first chunk (store); last: its last (scale).  Clears
+    // the accumulators.
+    // Column exponents of this lane's 16 results (clamped into the padded
+    // exponent arrays, so the loads issue together).
+    auto load_exps = [&](int (&sc)[16]) {
+        const int ea_i = ct.ea[min(ct.i0 + wi + lane % 32, ct.nc_pad - 1)];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) sc[r] = ct.eb[min(ct.j0 + wj + res_row(r), ct.ngp_pad - 1)] + ea_i - 12;
+    };
+    auto flush = [&](bool first, bool last) {
+        const int ig = ct.i0 + wi + lane % 32;
+        int sc[16];
+        if (last) load_exps(sc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int j = ct.j0 + wj + res_row(r);
+            if (j < ct.ngpown && ig < ct.ncouls) {
+                double sr, sm;
+                fold(r, sr, sm);
+                cplx *o = &ct.mout[(size_t)j * ct.ncouls + ig];
+                if (!first) {
+                    const cplx prev = *o;
+                    sr = prev.re + sr;
+                    sm = prev.im + sm;
+                }
+                if (last) {
+                    sr = ldexp(sr, sc[r]);
+                    sm = ldexp(sm, sc[r]);
+                }
+                *o = cplx{sr, sm};
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            acc_re[t] = acc_t{};
+            acc_im[t] = acc_t{};
+        }
+    };
+    bool drained = false;  // memory traffic since the last wait: wait for everything
+    for (;;) {
+        const int nst = ct.kpad / KS;
+        for (int c0 = 0; c0 < nst; c0 += CST) {
+            const int c1 = min(nst, c0 + CST);
+            for (int st = c0; st < c1; ++st, ++g) {
+                // Stages issued after g (PER_T loads each) may stay in flight.
+                const int ahead = drained ? 0 : gi - 1 - g;
+                if (ahead <= 0)
+                    wait_vmcnt<0>();
+                else if (ahead == 1)
+                    wait_vmcnt<PER_T>();
+                else
+                    wait_vmcnt<(NBUF > 3 ? 2 * PER_T : PER_T)>();
+                drained = false;
+                // Stage g is in LDS for every wave, and every wave is done
+                // with stage g - 1, whose buffer stage g + NBUF - 1 reuses.
+                barrier_no_fence();
+                const bool prefetch = pvalid && gi < g + NBUF;
+                const unsigned char *sB = lds[g % NBUF];
+                const unsigned char *sA = sB + SIDE;
+                frag fa[S][2];
+#pragma unroll
+                for (int q = 0; q < S; ++q)
+#pragma unroll
+                    for (int part = 0; part < 2; ++part)
+                        fa[q][part] = *reinterpret_cast<const frag *>(sA + q * PLANE + fo_a[part]);
+#pragma unroll
+                for (int p = 0; p < S; ++p) {
+                    const frag br = *reinterpret_cast<const frag *>(sB + p * PLANE + fo_b[0]);
+                    const frag bi = *reinterpret_cast<const frag *>(sB + p * PLANE + fo_b[1]);
+                    const frag bn = T::neg(bi);
+#pragma unroll
+                    for (int q = 0; q + p < S; ++q) {
+                        acc_re[p + q] = T::mfma32(br, fa[q][0], acc_re[p + q]);
+                        acc_im[p + q] = T::mfma32(br, fa[q][1], acc_im[p + q]);
+                    }
+                    if (prefetch) issue_units(2 * p, 2 * p + 2);
+#pragma unroll
+                    for (int q = 0; q + p < S; ++q) {
+                        acc_re[p + q] = T::mfma32(bi, fa[q][1], acc_re[p + q]);
+                        acc_im[p + q] = T::mfma32(bn, fa[q][0], acc_im[p + q]);
+                    }
+                }
+                if (prefetch) advance_issue();
+            }
+            if constexpr (CHUNKED) {
+                flush(c0 == 0, c1 == nst);
+                drained = true;
+            }
+        }
+        if constexpr (!CHUNKED) {
+            const int ig = ct.i0 + wi + lane % 32;
+            int sc[16];
+            load_exps(sc);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = ct.j0 + wj + res_row(r);
+                if (j < ct.ngpown && ig < ct.ncouls) {
+                    double sr, sm;
+                    fold(r, sr, sm);
+                    ct.mout[(size_t)j * ct.ncouls + ig] = cplx{ldexp(sr, sc[r]), ldexp(sm, sc[r])};
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                acc_re[t] = acc_t{};
+                acc_im[t] = acc_t{};
+            }
+            drained = true;
+        }
+        if (!src.get(++ck, ct)) break;
     }
 }
 

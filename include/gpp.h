@@ -108,8 +108,9 @@ typedef enum {
      * each band step through registers; 2 = 64x32 tiles, 2x2 waves of 32x16
      * on 16x16 MFMAs, band steps loaded straight into LDS, two blocks per
      * compute unit; 3 = 64x64 tiles, 2x2 waves of 32x32 on 32x32 MFMAs, band
-     * steps loaded straight into two to four LDS buffers ahead of use
-     * (default). */
+     * steps loaded straight into two to four LDS buffers ahead of use; 4 =
+     * layout 3 in persistent blocks, one per compute unit, whose tiles form
+     * one stream of band steps (default). */
     GPP_OPT_OZAKI_LAYOUT = 2,
     /* FP64-MFMA Gram tile (G' columns x G-vectors): 0 = 128x128 (default),
      * 1 = 64x128, 2 = 128x64, 3 = 64x64. */

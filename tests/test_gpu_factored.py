@@ -141,16 +141,16 @@ def test_ozaki_layouts_and_digit_types_agree(ctx, oracle, nb):
     ctx.upload(problem_of(arrs, 3), arrs)
     out = {}
     try:
-        for lay, group in ((0, 0), (1, 0), (2, 0), (2, 1), (2, 5), (3, 0), (3, 1)):
+        for lay, group in ((0, 0), (1, 0), (2, 0), (2, 1), (2, 5), (3, 0), (3, 1), (4, 0), (4, 1)):
             ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, lay)
             ctx.set_option(pygpp.GPP_OPT_OZAKI_GROUP, group)
             for form in ("factored-ozaki", "factored-ozaki-i8"):
                 out[(form, lay, group)] = ctx.sigma(form)
     finally:
-        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 3)
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 4)
         ctx.set_option(pygpp.GPP_OPT_OZAKI_GROUP, 0)
     for form in ("factored-ozaki", "factored-ozaki-i8"):
-        for key in [(0, 0), (1, 0), (2, 1), (2, 5), (3, 0), (3, 1)]:
+        for key in [(0, 0), (1, 0), (2, 1), (2, 5), (3, 0), (3, 1), (4, 0), (4, 1)]:
             for k in KEYS:
                 np.testing.assert_array_equal(out[(form,) + key][k], out[(form, 2, 0)][k])
     assert_close(out[("factored-ozaki", 2, 0)], out[("factored-ozaki-i8", 2, 0)], 1e-13, KEYS)
@@ -167,17 +167,18 @@ def test_ozaki_long_band_range(ctx, oracle):
     ctx.upload(problem_of(arrs, 2), arrs)
     ref = oracle.sigma(arrs, nvband=2, threads=8)
     out = {}
-    for lay in (3, 1):
+    for lay in (3, 1, 4):
         ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, lay)
         try:
             for form in ("factored-ozaki-i8", "factored-ozaki"):
                 out[(form, lay)] = ctx.sigma(form)
         finally:
-            ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 3)
+            ctx.set_option(pygpp.GPP_OPT_OZAKI_LAYOUT, 4)
     for form in ("factored-ozaki-i8", "factored-ozaki"):
         assert_close(out[(form, 3)], ref, TOL_FACTORED, KEYS)
         for k in KEYS:
             np.testing.assert_array_equal(out[(form, 1)][k], out[(form, 3)][k])
+            np.testing.assert_array_equal(out[(form, 4)][k], out[(form, 3)][k])
 
 
 def test_ozaki_option_validation(ctx):
@@ -185,11 +186,11 @@ def test_ozaki_option_validation(ctx):
     for bad in (2, 7):
         with pytest.raises(pygpp.GppError):
             ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, bad)
-    for opt, bad in ((pygpp.GPP_OPT_OZAKI_LAYOUT, 4), (pygpp.GPP_OPT_OZAKI_LAYOUT, -1),
+    for opt, bad in ((pygpp.GPP_OPT_OZAKI_LAYOUT, 5), (pygpp.GPP_OPT_OZAKI_LAYOUT, -1),
                      (pygpp.GPP_OPT_OZAKI_GROUP, -1)):
         with pytest.raises(pygpp.GppError):
             ctx.set_option(opt, bad)
-    assert ctx.get_option(pygpp.GPP_OPT_OZAKI_LAYOUT) == 3
+    assert ctx.get_option(pygpp.GPP_OPT_OZAKI_LAYOUT) == 4
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_GROUP) == 0
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
 
