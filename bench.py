@@ -315,7 +315,7 @@ def main():
             flop = 21 * 4 * 2 * NGPOWN * NCOULS * K
             i8 = form.endswith("i8")
             peak = PEAK_INT8_TOPS if i8 else PEAK_FP16_TFLOPS
-            # the default block layout (3) runs 32x32 MFMAs
+            # the default block layout (4, persistent layout 3) runs 32x32 MFMAs
             pname = ("v_mfma_i32_32x32x32_i8" if i8 else "v_mfma_f32_32x32x16_f16") + ", digit splits included"
             tr = None
         achieved = flop / (gms * 1e-3) / 1e12
