@@ -84,6 +84,28 @@ def test_batch_bitwise_equals_single_path(ctx):
             np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize("tile", [0, 3])
+def test_batch_gram_tiles_bitwise(ctx, tile):
+    """The batched FP64 Gram with the 128 x 128 (0) and 64 x 64 (3) tiles,
+    with an instance of several hundred tiles: bitwise equal to the
+    single-problem path with the same tile."""
+    import pygpp
+    probs = [pygpp.Problem(*s) for s in SHAPES] + [pygpp.Problem(7, 2, 30000, 300, 3)]
+    seeds = [41 + i for i in range(len(probs))]
+    ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, tile)
+    try:
+        with pygpp.Batch(ctx, probs, seeds=seeds) as b:
+            outs = b.sigma()
+            assert b.last_timing()["gram_tile"] == tile
+        for p, s, out in zip(probs, seeds, outs):
+            ctx.generate_seeded(p, s)
+            ref = ctx.sigma("factored-f64")
+            for k in KEYS:
+                np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
+
+
 @pytest.mark.parametrize("form", ["factored-ozaki-i8", "factored-ozaki"])
 def test_batch_ozaki_vs_oracle_and_single_path(ctx, oracle, form):
     """The Ozaki forms in a batch: every instance within 1e-10 of the oracle

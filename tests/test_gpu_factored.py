@@ -198,6 +198,39 @@ def test_ozaki_option_validation(ctx):
 ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 
+GRAM_TILE_SHAPES = [
+    (37, 2, 12000, 300, 3),   # K = 35: a partial last band step
+    (6, 2, 40000, 200, 3),    # 4 bands, fewer than one band step
+    (5, 5, 40000, 130, 3),    # no unoccupied band: Mu is zero
+    (130, 4, 300, 140, 3),    # partial tiles in both directions
+]
+
+
+@pytest.mark.parametrize("shape", GRAM_TILE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_gram_tiles_agree(ctx, oracle, shape):
+    """The four FP64 Gram tilings accumulate every Mu entry over the bands in
+    the same order: bit-identical outputs (with and without acht_n1_loc),
+    within 1e-10 of the oracle."""
+    import pygpp
+    nb, nv, ncouls, ngpown, nfreq = shape
+    arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=nb + ngpown)
+    ctx.upload(problem_of(arrs, nv), arrs)
+    out = {}
+    try:
+        for tile in range(4):
+            ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, tile)
+            out[tile] = ctx.sigma("factored-f64")
+            out[(tile, "acht")] = ctx.sigma("factored-f64", acht_n1_loc=True)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
+    for tile in range(1, 4):
+        for k in KEYS:
+            np.testing.assert_array_equal(out[tile][k], out[0][k], err_msg=f"tile {tile} {k}")
+        for k in KEYS + ("acht_n1_loc",):
+            np.testing.assert_array_equal(out[(tile, "acht")][k], out[(0, "acht")][k], err_msg=f"tile {tile} {k}")
+    assert_close(out[3], oracle.sigma(arrs, nvband=nv, threads=8), TOL_FACTORED, KEYS)
+
+
 def test_c0_cpu_config(ctx, oracle):
     """The CPU-runnable configuration: 512 2 4096 10 (ngpown 409), the
     reference's deterministic init, every form against the oracle; also the
