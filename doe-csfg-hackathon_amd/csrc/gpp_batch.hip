@@ -154,8 +154,9 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 }
 
 // Reduce items of instances whose occupied band count is NV (0: any count).
+// Occupancy as the single-problem reduce: two waves per SIMD at FC = 8.
 template <int FC, int NV>
-__global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__restrict__ insts,
+__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 4 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
                                                               const RedItem *__restrict__ items, int nitems,
                                                               int chunk)
 {
@@ -169,9 +170,8 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_reduce(const BInst *__res
         ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
         ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
-        ra.fpb = FC;
-        pole_reduce_item<FC, NV>(ra, it.j, it.split, it.iw0, it.iw0 + FC < b.nfreq ? it.iw0 + FC : b.nfreq,
-                                 scratch);
+        ra.fpb = b.nfreq;
+        pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC)>(ra, it.j, it.split, 0, b.nfreq, scratch);
     }
 }
 
@@ -240,14 +240,17 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         const long K = p.nb - p.nv;
         for (int i0 = 0; i0 < p.ncouls; i0 += ti)
             for (int j0 = 0; j0 < p.ngpown; j0 += tj) g.push_back({K, GramItem{i, j0, i0, 0}});
-        const int fc = k.red_fc == 4 ? 4 : 8;  // one FC for the whole launch; extra slots are masked
+        // One FC for the whole launch: an item runs all frequencies of its
+        // (column, G range) in passes of FC, the last one 4 wide when at most
+        // 4 remain (masked slots beyond nfreq still cost).
+        const int fc = k.red_fc == 4 ? 4 : 8;
+        const int slots = (p.nfreq / fc) * fc + (p.nfreq % fc == 0 ? 0 : p.nfreq % fc <= 4 ? 4 : fc);
         const int nsplit = pl.inst_off[i].nsplit;
-        for (int iw0 = 0; iw0 < p.nfreq; iw0 += fc)
-            for (int j = 0; j < p.ngpown; ++j)
-                for (int sp = 0; sp < nsplit; ++sp) {
-                    const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
-                    r[red_class(p.nv)].push_back({len * fc * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp, iw0}});
-                }
+        for (int j = 0; j < p.ngpown; ++j)
+            for (int sp = 0; sp < nsplit; ++sp) {
+                const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
+                r[red_class(p.nv)].push_back({len * slots * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp}});
+            }
         for (int n1 = 0; n1 < p.nb; ++n1) a.push_back(int2{i, n1});
     }
     auto by_cost = [](const auto &x, const auto &y) { return x.first > y.first; };

@@ -112,13 +112,13 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restr
 
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
 // compile time (0: any).
-template <int FC, int NV>
+template <int FC, int NV, bool P2>
 __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 4 : 2) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
     const int iw_end = iw_begin + a.fpb < a.nfreq ? iw_begin + a.fpb : a.nfreq;
-    pole_reduce_item<FC, NV>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
+    pole_reduce_item<FC, NV, P2, (FC > 4 ? 4 : FC)>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach, const cplx *col_asx,
@@ -215,17 +215,27 @@ void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t 
                        p.nb, mu);
 }
 
+// P2: also store the pole factor (and occupied branch) of frequency acht_iw.
+template <int FC, bool P2>
+void launch_reduce_p2(const ReduceArgs &a, dim3 grid, hipStream_t s)
+{
+    switch (a.nv) {  // the occupied band counts of the common shapes
+    case 2: hipLaunchKernelGGL((k_pole_reduce<FC, 2, P2>), grid, dim3(kRedThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_pole_reduce<FC, 4, P2>), grid, dim3(kRedThreads), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_pole_reduce<FC, 8, P2>), grid, dim3(kRedThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_pole_reduce<FC, 0, P2>), grid, dim3(kRedThreads), 0, s, a); break;
+    }
+}
+
 template <int FC>
 void launch_reduce_fc(ReduceArgs a, hipStream_t s)
 {
     if (a.fpb < FC) a.fpb = FC;
     dim3 grid(a.ngpown, a.nsplit, (a.nfreq + a.fpb - 1) / a.fpb);
-    switch (a.nv) {  // the occupied band counts of the common shapes
-    case 2: hipLaunchKernelGGL((k_pole_reduce<FC, 2>), grid, dim3(kRedThreads), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_pole_reduce<FC, 4>), grid, dim3(kRedThreads), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_pole_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_pole_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, a); break;
-    }
+    if (a.p2)
+        launch_reduce_p2<FC, true>(a, grid, s);
+    else
+        launch_reduce_p2<FC, false>(a, grid, s);
 }
 
 #define GPP_TRY(x)                       \

@@ -11,6 +11,8 @@
 // ssxt_scht_solver (:16-63); sch = P in branch 1 and 0 otherwise.
 #pragma once
 
+#include <type_traits>
+
 #include "gpp_internal.hpp"
 
 namespace gpp {
@@ -175,7 +177,7 @@ struct ReduceArgs {
 // NV > 0: the occupied band count is NV (== a.nv), known at compile time, so
 // the occupied loads are unrolled and issued together and conj(bg) of the
 // column is loaded once; NV == 0: any a.nv.
-template <int FC, int NV = 0>
+template <int FC, int NV = 0, bool P2 = false, int FCT = FC>
 __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw_begin, int iw_end,
                                                  cplx *scratch)
 {
@@ -186,6 +188,10 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
     const double limitone = 1.0 / (kTo1 * 4.0);
+    // 2^-40 bands around limitone (:22) and to1 (:20) for the |delw|^2 tests
+    constexpr double kBand = 1.0 / 1099511627776.0;
+    constexpr double kEstLo1 = 1.0 / (kTo1 * 4.0) * (1 - kBand), kEstHi1 = 1.0 / (kTo1 * 4.0) * (1 + kBand);
+    constexpr double kEstLo2 = kTo1 * (1 - kBand), kEstHi2 = kTo1 * (1 + kBand);
     // conj(Bg[n1][j]) of the occupied bands, loaded once for the column (the
     // optional p2/br1 stores in the loop would otherwise keep the compiler
     // from hoisting these loads out of it).
@@ -194,101 +200,129 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 #pragma unroll
         for (int n1 = 0; n1 < NV; ++n1) bgc[n1] = cconj(a.bg[(size_t)n1 * a.ngpown + j]);
     }
-    // Frequencies in passes of FC; the passes after the first re-read this
-    // block's G-vector range, recently fetched, from the caches.
-    for (int iw0 = iw_begin; iw0 < iw_end; iw0 += FC) {
-    double wxv[FC], c1[FC];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
+    // Frequencies in passes of FC, the last one FCT wide when at most FCT
+    // remain; the passes after the first re-read this block's G-vector range,
+    // recently fetched, from the caches.
+    auto pass = [&](auto width, int iw0) {
+        constexpr int F = decltype(width)::value;
+        double wxv[F], c1[F];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-        wxv[f] = (iw0 + f < iw_end) ? a.wx[iw0 + f] : 1.0;
-        const double wx2 = wxv[f] * wxv[f];
-        c1[f] = wx2 * (1.00 / (wx2 * wx2));
-    }
-    cplx sach[FC], sasx[FC];
-#pragma unroll
-    for (int f = 0; f < FC; ++f) sach[f] = sasx[f] = czero();
-
-    bool any_neg = false;  // the ssx cutoff of :58-59 applies only at negative frequencies
-#pragma unroll
-    for (int f = 0; f < FC; ++f) any_neg = any_neg || wxv[f] < 0.00;
-
-    const size_t row = (size_t)j * ncouls;
-    for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
-        const cplx wt = a.wtilde[row + ig];
-        const cplx ie = a.ieps[row + ig];
-        const cplx mu = a.mu[row + ig];
-        cplx mo = czero(), om2 = czero(), mum = mu, qh = czero();
-        double ie_cut = 0.0, wti2 = 0.0;
-        if (nv > 0) {
-            if constexpr (NV > 0) {
-#pragma unroll
-                for (int n1 = 0; n1 < NV; ++n1)  // occupied bands, in band order
-                    mo = cadd(mo, cmul_fast(bgc[n1], a.aqsn[(size_t)n1 * ncouls + ig]));
-            } else {
-                for (int n1 = 0; n1 < nv; ++n1)
-                    mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig]));
-            }
-            om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
-            mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
-            qh = cscale(cmul_fast(ie, wt), 0.5);
-            wti2 = wt.im * wt.im;
-            if (any_neg) {
-                // A real (uniform) branch: the empty asm keeps the compiler
-                // from evaluating the square root for every element.
-                asm volatile("" ::: "memory");
-                ie_cut = cabs_(ie) * kSexcut;
-            }
+        for (int f = 0; f < F; ++f) {
+            wxv[f] = (iw0 + f < iw_end) ? a.wx[iw0 + f] : 1.0;
+            const double wx2 = wxv[f] * wxv[f];
+            c1[f] = wx2 * (1.00 / (wx2 * wx2));
         }
+        cplx sach[F], sasx[F];
 #pragma unroll
-        for (int f = 0; f < FC; ++f) {
-            const cplx wdiff = cdminus(wxv[f], wt);
-            const double wdiffr = cmul(wdiff, cconj(wdiff)).re;
-            const double rden = rcp_refined(wdiffr);
-            const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);
-            const cplx p = cmul_fast(delw, ie);
-            cplx m = mu;
-            bool branch1 = false;
+        for (int f = 0; f < F; ++f) sach[f] = sasx[f] = czero();
+
+        bool any_neg = false;  // the ssx cutoff of :58-59 applies only at negative frequencies
+#pragma unroll
+        for (int f = 0; f < F; ++f) any_neg = any_neg || wxv[f] < 0.00;
+
+        const size_t row = (size_t)j * ncouls;
+        for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
+            const cplx wt = a.wtilde[row + ig];
+            const cplx ie = a.ieps[row + ig];
+            const cplx mu = a.mu[row + ig];
+            cplx mo = czero(), om2 = czero(), mum = mu;
+            double ie_cut = 0.0, wt2 = 0.0;
+            // P = delw I_eps = (wt I_eps) conj(wdiff) / |wdiff|^2 with conj(wdiff)
+            // = (wx - wt.re, wt.im): two FMAs on q = wt I_eps per frequency.
+            const cplx q = cmul_fast(wt, ie);
+            const double q_im_wti = q.im * wt.im, q_re_wti = q.re * wt.im;
+            // wdiff.im^2 rounded as the reference's |wdiff|^2 (:27, :33) rounds it
+            const double wti2 = wt.im * wt.im;
             if (nv > 0) {
-                const double delwr = cmul(delw, cconj(delw)).re;
-                cplx ssx = czero();
-                if ((wdiffr > 0.25) && (delwr < limitone)) {
-                    branch1 = true;
-                    m = mum;
-                    ssx = cscale(om2, c1[f]);
-                } else if (delwr > kTo1) {
-                    // :46-50 with cden = 4 wt^2 (0.5 + delw), Omega2 = wt^2 I_eps
-                    // and delw = wt / (wx - wt):
-                    //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
-                    //                                     = -(I_eps wt / 2) / (wx + wt),
-                    // a pole at -wt, far from cancellation for wx > 0.
-                    const double s = wxv[f] + wt.re;
-                    const double rr = rcp_1nr(__builtin_fma(s, s, wti2));
-                    ssx = cplx{-__builtin_fma(qh.re, s, qh.im * wt.im) * rr,
-                               -__builtin_fma(qh.im, s, -(qh.re * wt.im)) * rr};
-                }
-                if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
-                sasx[f] = cfma(ssx, mo, sasx[f]);
-            }
-            sach[f] = cfma(p, m, sach[f]);
-            if (a.p2 && iw0 + f == a.acht_iw && iw0 + f < iw_end) {
-                a.p2[row + ig] = p;
-                if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
-            }
-        }
-    }
-    const double vc = a.vcoul[igp_of(a.idx, j, ncouls)];
+                if constexpr (NV > 0) {
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-        const cplx s = block_sum(sach[f], scratch);
-        const cplx x = block_sum(sasx[f], scratch);
-        if (threadIdx.x == 0 && iw0 + f < iw_end) {
-            // (0.5 * sum) * vcoul and sum * (occ * vcoul), gppComplex.cpp:361-363, :379
-            const size_t o = ((size_t)j * a.nsplit + split) * a.nfreq + iw0 + f;
-            a.col_ach[o] = cscale(cscale(s, 0.5), vc);
-            a.col_asx[o] = cscale(x, a.occ * vc);
+                    for (int n1 = 0; n1 < NV; ++n1)  // occupied bands, in band order
+                        mo = cadd(mo, cmul_fast(bgc[n1], a.aqsn[(size_t)n1 * ncouls + ig]));
+                } else {
+                    for (int n1 = 0; n1 < nv; ++n1)
+                        mo = cadd(mo, cmul_fast(cconj(a.bg[(size_t)n1 * a.ngpown + j]),
+                                                a.aqsn[(size_t)n1 * ncouls + ig]));
+                }
+                om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
+                mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
+                wt2 = __builtin_fma(wt.re, wt.re, wti2);
+                if (any_neg) {
+                    // A real (uniform) branch: the empty asm keeps the compiler
+                    // from evaluating the square root for every element.
+                    asm volatile("" ::: "memory");
+                    ie_cut = cabs_(ie) * kSexcut;
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const double wdr = wxv[f] - wt.re;        // wdiff = (wdr, -wt.im), :27
+                const double wdiffr = wdr * wdr + wti2;   // |wdiff|^2 exactly as the reference rounds it
+                const double rden = rcp_1nr(wdiffr);
+                const cplx p = {__builtin_fma(q.re, wdr, -q_im_wti) * rden, __builtin_fma(q.im, wdr, q_re_wti) * rden};
+                cplx m = mu;
+                bool branch1 = false;
+                if (nv > 0) {
+                    // The branch tests of :35 and :43 on |delw|^2.  est = |wt|^2 /
+                    // |wdiff|^2 is within 1e-14 of the reference's rounded |delw|^2,
+                    // relative to it (each rounding of delw is bounded by |delw|,
+                    // with or without cancellation), so outside a 2^-40 band
+                    // around the two thresholds it decides as the reference does;
+                    // inside it, and for a non-finite est, the reference's delw
+                    // and |delw|^2 are recomputed in its operation order with the
+                    // correctly rounded reciprocal (:30-32).
+                    const double est = wt2 * rden;
+                    const bool sure = (est < kEstLo2) | ((est > kEstHi2) & (est < kEstLo1)) |
+                                      ((est > kEstHi1) & (est <= __DBL_MAX__));
+                    bool below1 = est < kEstLo1, above2 = est > kEstHi2;
+                    if (!sure) {
+                        const cplx wdiff = cdminus(wxv[f], wt);
+                        const cplx delw = cscale(cmul(wt, cconj(wdiff)), rcp_refined(wdiffr));
+                        const double delwr = cmul(delw, cconj(delw)).re;
+                        below1 = delwr < limitone;
+                        above2 = delwr > kTo1;
+                    }
+                    cplx ssx = czero();
+                    if ((wdiffr > 0.25) && below1) {
+                        branch1 = true;
+                        m = mum;
+                        ssx = cscale(om2, c1[f]);
+                    } else if (above2) {
+                        // :46-50 with cden = 4 wt^2 (0.5 + delw), Omega2 = wt^2 I_eps
+                        // and delw = wt / (wx - wt):
+                        //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
+                        //                                     = -(I_eps wt / 2) / (wx + wt),
+                        // a pole at -wt, far from cancellation for wx > 0.
+                        const double s = wxv[f] + wt.re;
+                        const double rr = 0.5 * rcp_1nr(__builtin_fma(s, s, wti2));
+                        ssx = cplx{-__builtin_fma(q.re, s, q.im * wt.im) * rr,
+                                   -__builtin_fma(q.im, s, -(q.re * wt.im)) * rr};
+                    }
+                    if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
+                    sasx[f] = cfma(ssx, mo, sasx[f]);
+                }
+                sach[f] = cfma(p, m, sach[f]);
+                if (P2 && iw0 + f == a.acht_iw && iw0 + f < iw_end) {
+                    a.p2[row + ig] = p;
+                    if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
+                }
+            }
         }
-    }
-    }
+        const double vc = a.vcoul[igp_of(a.idx, j, ncouls)];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            const cplx s = block_sum(sach[f], scratch);
+            const cplx x = block_sum(sasx[f], scratch);
+            if (threadIdx.x == 0 && iw0 + f < iw_end) {
+                // (0.5 * sum) * vcoul and sum * (occ * vcoul), gppComplex.cpp:361-363, :379
+                const size_t o = ((size_t)j * a.nsplit + split) * a.nfreq + iw0 + f;
+                a.col_ach[o] = cscale(cscale(s, 0.5), vc);
+                a.col_asx[o] = cscale(x, a.occ * vc);
+            }
+        }
+    };
+    int iw0 = iw_begin;
+    for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), iw0);
+    if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), iw0);
 }
 
 // acht_n1_loc (gppComplex.cpp:381, sch_array[2] of each band and column):

@@ -147,8 +147,8 @@ hipError_t launch_gram_ozaki(const DevProblem &p, int slices, bool int8, int lay
 struct GramItem {  // one Gram tile of one instance
     int inst, j0, i0, pad;
 };
-struct RedItem {   // one pole-reduce item of one instance
-    int inst, j, split, iw0;
+struct RedItem {   // one pole-reduce item of one instance: column j, G range split, every frequency
+    int inst, j, split;
 };
 
 // Workspace layout and work lists of one batch for one GramKind.

@@ -74,6 +74,59 @@ def near_pole_inputs(nb, ncouls, ngpown, seed=0):
     return a
 
 
+def ref_delwr(wx, wt):
+    """|delw|^2 and |wdiff|^2 of the occupied-band solver in the reference's
+    operation order (gppComplex.cpp:27-33: wdiff = wx - wt, rden = 1/|wdiff|^2,
+    delw = wt conj(wdiff) rden), one IEEE operation at a time."""
+    wtr, wti = np.real(wt), np.imag(wt)
+    wdr, wdi = wx - wtr, 0.0 - wti
+    cre, cim = wdr, -1.0 * wdi
+    wdiffr = wdr * cre - wdi * cim
+    rden = 1.0 / wdiffr
+    dre = (wtr * cre - wti * cim) * rden
+    dim_ = (wtr * cim + wti * cre) * rden
+    return dre * dre - dim_ * (-1.0 * dim_), wdiffr
+
+
+def threshold_inputs(nb, ncouls, ngpown, seed=0, spread=12):
+    """Seeded inputs with wx = (399.4, 4.0, 0.1) and wtilde placed so that the
+    reference's |delw|^2 (gppComplex.cpp:32) lies within a few ulps of a
+    branch threshold for one of the frequencies:
+      kind 0  |delw|^2 ~ limitone = 2.5e5 at wx = 399.4, |wdiff|^2 ~ 0.64 > 0.25:
+              branch 1 (:35) or branch 2 (:43) by the last bits
+      kind 1  |delw|^2 ~ to1 = 1e-6 at wx = 0.1, |wdiff|^2 ~ 0.01 <= 0.25:
+              branch 2 (:43) or neither by the last bits
+      kind 2  default seeded wtilde
+    Each threshold element's modulus is scaled by (1 + k 2^-52), k uniform in
+    [-spread, spread], so the decisions fall on both sides."""
+    a = seeded_inputs(nb, ncouls, ngpown, 3, seed)
+    rng = np.random.default_rng(seed + 2000)
+    shape = a["wtilde_array"].shape
+    kind = rng.integers(0, 3, shape)
+    theta = rng.uniform(-np.pi, np.pi, shape)
+    jitter = 1.0 + rng.integers(-spread, spread + 1, shape) * 2.0 ** -52
+
+    def at_ratio(wx, t, near_wx):
+        # wt with |wt|^2 / |wx - wt|^2 = t; near_wx: wt = wx + d e^{i theta}
+        # (pole side), else wt = r e^{i theta} (small wt)
+        c = np.cos(theta)
+        if near_wx:
+            # |wx + d e^{i th}|^2 = t d^2  ->  (t - 1) d^2 - 2 wx c d - wx^2 = 0
+            dd = (2 * wx * c + np.sqrt(4 * wx * wx * c * c + 4 * (t - 1) * wx * wx)) / (2 * (t - 1))
+            return wx + dd * np.exp(1j * theta)
+        # r^2 = t |wx - r e^{i th}|^2  ->  (1 - t) r^2 + 2 t wx c r - t wx^2 = 0
+        r = (-2 * t * wx * c + np.sqrt(4 * t * t * wx * wx * c * c + 4 * (1 - t) * t * wx * wx)) / (2 * (1 - t))
+        return r * np.exp(1j * theta)
+
+    wt = a["wtilde_array"]
+    wt = np.where(kind == 0, at_ratio(399.4, 2.5e5, True) * jitter, wt)
+    wt = np.where(kind == 1, at_ratio(0.1, 1e-6, False) * jitter, wt)
+    a["wtilde_array"] = wt
+    a["wx_array"] = np.array([399.4, 4.0, 0.1])
+    a["_kind"] = kind
+    return a
+
+
 def per_band_wx(nb, nfreq, seed=0, levels=None, e_lk=10.0, dw=1.0):
     """Band-dependent frequencies wx[n1][iw] = e_lk - e_n1kq[n1] + dw*(iw-1),
     clamped at 1e-6 as gppComplex.cpp:326-328 clamps its one row; BerkeleyGW

@@ -42,6 +42,21 @@ def test_factored_near_pole(ctx, oracle, form):
     assert_close(run_gpu(ctx, arrs, 4, form), oracle.sigma(arrs, nvband=4, threads=8), TOL_FACTORED, KEYS)
 
 
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_branch_thresholds(ctx, oracle, form):
+    """Two thirds of the elements put the reference's |delw|^2 within a few
+    ulps of limitone (wx = 399.4) or to1 (wx = 0.1), on both sides.  The
+    factored forms decide gppComplex.cpp:35 and :43 from an estimate of
+    |delw|^2 away from the thresholds and from the reference's own rounding
+    near them: every decision, and so every output, matches the oracle (a
+    single wrong decision moves an output by far more than 1e-10)."""
+    arrs = gpp_inputs.threshold_inputs(24, 2000, 60, seed=5)
+    arrs.pop("_kind")
+    ref = oracle.sigma(arrs, nvband=4, threads=8)
+    assert_close(run_gpu(ctx, arrs, 4, form), ref, TOL_FACTORED, KEYS)
+    assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
+
+
 @pytest.mark.parametrize("ncouls", [800, 3200])
 @pytest.mark.parametrize("form", FACTORED)
 def test_factored_readme_golden(ctx, oracle, ncouls, form):

@@ -108,6 +108,20 @@ def _check_exact(oracle, arrs, nv, tol=1e-13):
         assert abs(out["acht_n1_loc"][n1] - e) <= tol * (ex["achtemp_mag"][-1] + ex["achtemp_mag"][min(2, len(arrs["wx_array"]) - 1)])
 
 
+def test_threshold_inputs_straddle_thresholds():
+    """The threshold inputs the factored-form tests use put the reference's
+    |delw|^2 within 2^-40 of limitone and of to1 for thousands of elements,
+    on both sides of each threshold, and keep |wdiff|^2 on the side of 0.25
+    that makes the |delw|^2 test decide (gppComplex.cpp:35, :43)."""
+    a = gpp_inputs.threshold_inputs(24, 2000, 60, seed=5)
+    kind, wt = a["_kind"], a["wtilde_array"]
+    for k, wx, t, above_quarter in ((0, 399.4, 2.5e5, True), (1, 0.1, 1e-6, False)):
+        delwr, wdiffr = gpp_inputs.ref_delwr(wx, wt[kind == k])
+        band = np.abs(delwr / t - 1) < 2.0 ** -40
+        assert (delwr[band] < t).sum() > 1000 and (delwr[band] > t).sum() > 1000
+        assert np.all((wdiffr > 0.25) == above_quarter)
+
+
 def test_exact_rational_det(oracle):
     _check_exact(oracle, gpp_inputs.det_inputs(4, 2, 6, ngpown=3), 2)
 
