@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_achstemp(const BInst *__r
 }
 
 template <int WJN, int WIN>
-__global__ __launch_bounds__(64 * WJN * WIN) void k_batch_gram(const BInst *__restrict__ insts,
+__global__ __launch_bounds__(64 * WJN * WIN, 8 / (WJN * WIN)) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_gram(const BInst *__restrict__ insts,
                                                                const GramItem *__restrict__ items, int nitems)
 {
     typedef GramCfg<WJN, WIN> C;

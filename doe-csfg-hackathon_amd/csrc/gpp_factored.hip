@@ -93,9 +93,10 @@ __global__ void k_gather_aqsm(const cplx *__restrict__ aqsm, const int32_t *__re
 
 // 1-D grid over the ceil(ngpown/TJ) x ceil(ncouls/TI) tiles, G' tiles
 // fastest in tile order and tiles spread over the XCDs by xcd_tile, so the
-// blocks sharing one aqsn tile run on one XCD.
+// blocks sharing one aqsn tile run on one XCD.  Every tile runs two waves per
+// SIMD (at most 256 registers per lane).
 template <int WJN, int WIN>
-__global__ __launch_bounds__(64 * WJN * WIN) void k_gram_f64(const cplx *__restrict__ bg,
+__global__ __launch_bounds__(64 * WJN * WIN, 8 / (WJN * WIN)) __attribute__((amdgpu_waves_per_eu(2))) void k_gram_f64(const cplx *__restrict__ bg,
                                                                     const cplx *__restrict__ aqsn, int ngpown,
                                                                     int ncouls, int k_begin, int k_end,
                                                                     cplx *__restrict__ mout)

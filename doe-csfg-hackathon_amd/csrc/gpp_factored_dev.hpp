@@ -42,6 +42,12 @@ __device__ __forceinline__ void gather_elem(const cplx *aqsm, const int32_t *idx
 // B[l/16][l%16], result r of lane l = C[4r + l/16][l%16]; MFMA rows are G'
 // columns j and MFMA columns are G-vectors ig, so each result row is
 // contiguous in ig.  sB/sA are the block's double-buffered LDS stages.
+// Three real products per complex product (3M): with b = Bg, a = aqsn,
+//   P1 = sum br ar,  P2 = sum bi ai,  P3 = sum (br - bi)(ar + ai),
+//   Re Mu = P1 + P2,  Im Mu = (P3 - P1) + P2  (= sum br ai - bi ar),
+// three accumulators fed by one MFMA each per k-step instead of two fed by
+// two.  Rounding: |P3| <= 2 sum |b||a|, so the error stays of order
+// 1e-16 sum |b||a| (normwise), as with four products.
 template <int WJN, int WIN>
 __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
                                               int ncouls, int k_begin, int k_end, int j0, int i0,
@@ -57,14 +63,11 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
     const int wj = (wave / WIN) * kWJ;
     const int wi = (wave % WIN) * kWI;
 
-    v4d acc_re[4][2], acc_im[4][2];
+    v4d acc1[4][2], acc2[4][2], acc3[4][2];  // P1, P2, P3
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            acc_re[s][t] = v4d{0, 0, 0, 0};
-            acc_im[s][t] = v4d{0, 0, 0, 0};
-        }
+        for (int t = 0; t < 2; ++t) acc1[s][t] = acc2[s][t] = acc3[s][t] = v4d{0, 0, 0, 0};
 
     cplx stB[PB], stA[PA];
     auto load_stage = [&](int k0) {
@@ -104,30 +107,27 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
 #pragma unroll
         for (int kk = 0; kk < kKS; kk += 4) {
             const int kr = kk + lane / 16;
-            double bre[4], bim[4], bneg[4], are[2], aim[2];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const cplx b = sB[buf][kr][wj + 16 * s + lane % 16];
-                bre[s] = b.re;
-                bim[s] = b.im;
-                bneg[s] = -b.im;
-            }
+            double are[2], aim[2], asum[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const cplx a = sA[buf][kr][wi + 16 * t + lane % 16];
                 are[t] = a.re;
                 aim[t] = a.im;
+                asum[t] = a.re + a.im;
             }
-            // conj(b) * a: re = br*ar + bi*ai, im = br*ai - bi*ar
+            // one B sub-tile at a time keeps the fragments within the
+            // register budget of two waves per SIMD
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s) {
+                const cplx b = sB[buf][kr][wj + 16 * s + lane % 16];
+                const double bdiff = b.re - b.im;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[s], are[t], acc_re[s][t], 0, 0, 0);
-                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim[s], aim[t], acc_re[s][t], 0, 0, 0);
-                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[s], aim[t], acc_im[s][t], 0, 0, 0);
-                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bneg[s], are[t], acc_im[s][t], 0, 0, 0);
+                    acc1[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.re, are[t], acc1[s][t], 0, 0, 0);
+                    acc2[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.im, aim[t], acc2[s][t], 0, 0, 0);
+                    acc3[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bdiff, asum[t], acc3[s][t], 0, 0, 0);
                 }
+            }
         }
         if (st + 1 < nst) store_stage(buf ^ 1);
         __syncthreads();
@@ -141,7 +141,9 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
             for (int r = 0; r < 4; ++r) {
                 const int j = j0 + wj + 16 * s + 4 * r + lane / 16;
                 const int ig = i0 + wi + 16 * t + lane % 16;
-                if (j < ngpown && ig < ncouls) mout[(size_t)j * ncouls + ig] = cplx{acc_re[s][t][r], acc_im[s][t][r]};
+                if (j < ngpown && ig < ncouls)
+                    mout[(size_t)j * ncouls + ig] =
+                        cplx{acc1[s][t][r] + acc2[s][t][r], (acc3[s][t][r] - acc1[s][t][r]) + acc2[s][t][r]};
             }
 }
 
@@ -332,7 +334,9 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 // (acht_occ_item below: few bands, a streaming pass).  For the unoccupied
 // bands Y = R * aqsn^T is a second Gram-type product (rows j, columns n1, K = G);
 // one block computes a TJ x TN tile of Y over G-vectors [k_begin, k_end) on
-// v_mfma_f64_16x16x4f64 and reduces it at once to the column sums
+// v_mfma_f64_16x16x4f64 with three real products per complex product (3M:
+// P1 = sum rr ar, P2 = sum ri ai, P3 = sum (rr + ri)(ar + ai), Re Y = P1 - P2,
+// Im Y = (P3 - P1) - P2) and reduces it at once to the column sums
 // sum_j w_j conj(bg[n1][j]) Y[j][n1], written to part[n1] (one entry per
 // column of the tile).  Operand tiles are staged K-fastest in 128-byte rows
 // of 8 complex, element (row, k) at slot k ^ (row & 7): the stores (8 lanes
@@ -364,14 +368,11 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
     const int wj = wjn * kWJ;
     const int wi = (wave % WIN) * kWI;
 
-    v4d acc_re[4][2], acc_im[4][2];
+    v4d acc1[4][2], acc2[4][2], acc3[4][2];  // P1, P2, P3
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            acc_re[s][t] = v4d{0, 0, 0, 0};
-            acc_im[s][t] = v4d{0, 0, 0, 0};
-        }
+        for (int t = 0; t < 2; ++t) acc1[s][t] = acc2[s][t] = acc3[s][t] = v4d{0, 0, 0, 0};
 
     cplx stR[PR], stA[PA];
     auto load_stage = [&](int k0) {
@@ -413,30 +414,25 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
 #pragma unroll
         for (int kk = 0; kk < kAK; kk += 4) {
             const int kr = (kk + lane / 16) ^ (lane & 7);  // every row read below is = lane (mod 8)
-            double rre[4], rim[4], rneg[4], are[2], aim[2];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const cplx r = sR[buf][wj + 16 * s + lane % 16][kr];
-                rre[s] = r.re;
-                rim[s] = r.im;
-                rneg[s] = -r.im;
-            }
+            double are[2], aim[2], asum[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const cplx a = sA[buf][wi + 16 * t + lane % 16][kr];
                 are[t] = a.re;
                 aim[t] = a.im;
+                asum[t] = a.re + a.im;
             }
-            // R * a: re = rr*ar - ri*ai, im = rr*ai + ri*ar
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s) {
+                const cplx r = sR[buf][wj + 16 * s + lane % 16][kr];
+                const double rsum = r.re + r.im;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rre[s], are[t], acc_re[s][t], 0, 0, 0);
-                    acc_re[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rneg[s], aim[t], acc_re[s][t], 0, 0, 0);
-                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rre[s], aim[t], acc_im[s][t], 0, 0, 0);
-                    acc_im[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rim[s], are[t], acc_im[s][t], 0, 0, 0);
+                    acc1[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(r.re, are[t], acc1[s][t], 0, 0, 0);
+                    acc2[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(r.im, aim[t], acc2[s][t], 0, 0, 0);
+                    acc3[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(rsum, asum[t], acc3[s][t], 0, 0, 0);
                 }
+            }
         }
         if (st + 1 < nst) store_stage(buf ^ 1);
         __syncthreads();
@@ -459,7 +455,7 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
                 if (j < ngpown && n1 < n1_end) {
                     const double w = 0.5 * vcoul[igp_of(idx, j, ncouls)];
                     const cplx b = bg[(size_t)n1 * ngpown + j];
-                    const cplx y = cplx{acc_re[s][t][r], acc_im[s][t][r]};
+                    const cplx y = cplx{acc1[s][t][r] - acc2[s][t][r], (acc3[s][t][r] - acc1[s][t][r]) - acc2[s][t][r]};
                     sum = cadd(sum, cscale(cmul_fast(cconj(b), y), w));
                 }
             }
