@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kRedThreads) void k_batch_achstemp(const BInst *__r
 }
 
 template <int WJN, int WIN>
-__global__ __launch_bounds__(64 * WJN * WIN, 8 / (WJN * WIN)) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_gram(const BInst *__restrict__ insts,
+__global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 2) void k_batch_gram(const BInst *__restrict__ insts,
                                                                const GramItem *__restrict__ items, int nitems)
 {
     typedef GramCfg<WJN, WIN> C;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 // Reduce items of instances whose occupied band count is NV (0: any count).
 // Occupancy as the single-problem reduce: two waves per SIMD at FC = 8.
 template <int FC, int NV>
-__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 4 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
+__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
                                                               const RedItem *__restrict__ items, int nitems,
                                                               int chunk)
 {

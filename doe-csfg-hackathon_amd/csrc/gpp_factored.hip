@@ -96,7 +96,7 @@ __global__ void k_gather_aqsm(const cplx *__restrict__ aqsm, const int32_t *__re
 // blocks sharing one aqsn tile run on one XCD.  Every tile runs two waves per
 // SIMD (at most 256 registers per lane).
 template <int WJN, int WIN>
-__global__ __launch_bounds__(64 * WJN * WIN, 8 / (WJN * WIN)) __attribute__((amdgpu_waves_per_eu(2))) void k_gram_f64(const cplx *__restrict__ bg,
+__global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 2) void k_gram_f64(const cplx *__restrict__ bg,
                                                                     const cplx *__restrict__ aqsn, int ngpown,
                                                                     int ncouls, int k_begin, int k_end,
                                                                     cplx *__restrict__ mout)
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(64 * WJN * WIN, 8 / (WJN * WIN)) __attribute__((amd
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
 // compile time (0: any).
 template <int FC, int NV, bool P2>
-__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 4 : 2) void k_pole_reduce(ReduceArgs a)
+__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
