@@ -64,7 +64,8 @@ def cpu_baseline():
     to the oracle's race-free OpenMP restatement when the reference binary is
     absent.  Runs before the GPU is initialised."""
     cores = cpu_cores()
-    nb_s = 320  # about 10 s on 16 host cores
+    # about 5e10 updates (10 s on 16 host cores): 305 bands at c1, 77 at c3
+    nb_s = min(NB, max(16, int(5e10 / (NGPOWN * NCOULS * 3))))
     nv_s = int(round(NV * nb_s / NB))  # keep the occupied-band fraction of the full problem
     ref_exe = os.path.join(ORACLE, "_ref", "gppOpenMP3.ex")
     # the reference fixes nfreq = 3 (gppComplex.cpp:12-13); its rate per update is what is compared
