@@ -205,8 +205,9 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     // Frequencies in passes of FC, the last one FCT wide when at most FCT
     // remain; the passes after the first re-read this block's G-vector range,
     // recently fetched, from the caches.
-    auto pass = [&](auto width, int iw0) {
+    auto pass = [&](auto width, auto fold, int iw0) {
         constexpr int F = decltype(width)::value;
+        constexpr bool FOLD = decltype(fold)::value;
         double wxv[F], c1[F];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
 #pragma unroll
         for (int f = 0; f < F; ++f) {
@@ -223,11 +224,9 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         for (int f = 0; f < F; ++f) any_neg = any_neg || wxv[f] < 0.00;
 
         const size_t row = (size_t)j * ncouls;
-        // With passes of at most 3 frequencies the next element's wtilde,
-        // I_eps and Mu load while this one is evaluated (two elements in
-        // flight per thread, 6 % faster at 3 frequencies); the wider kernels
-        // have no registers to spare for it at their occupancy.
-        constexpr bool PREFETCH = FC <= 3;
+        // The next element's wtilde, I_eps and Mu load while this one is
+        // evaluated (two elements in flight per thread).
+        constexpr bool PREFETCH = true;
         cplx wt_n = czero(), ie_n = czero(), mu_n = czero();
         if (PREFETCH && ig_begin + (int)threadIdx.x < ig_end) {
             wt_n = a.wtilde[row + ig_begin + threadIdx.x];
@@ -250,14 +249,29 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 ie = a.ieps[row + ig];
                 mu = a.mu[row + ig];
             }
-            cplx mo = czero(), om2 = czero(), mum = mu;
-            double ie_cut = 0.0, wt2 = 0.0;
-            // P = delw I_eps = (wt I_eps) conj(wdiff) / |wdiff|^2 with conj(wdiff)
-            // = (wx - wt.re, wt.im): two FMAs on q = wt I_eps per frequency.
+            // P = delw I_eps = q conj(wdiff) / |wdiff|^2 with q = wt I_eps and
+            // conj(wdiff) = (wx - wt.re, wt.im) =: (wdr, wt.im).  FOLD (problems
+            // with 4 or more frequencies) folds Mu and Mo into the products
+            // once per element -- u = q Mu (every element), uo = q Mo (branch
+            // 1: sch = P for the occupied bands) and Omega2 Mo (branch 1's ssx
+            // Mo = c1 Omega2 Mo) -- so each frequency adds u conj(wdiff) /
+            // |wdiff|^2 with four FMAs (C4 reduce 18.2 -> 14.9 ms); with fewer
+            // frequencies P is formed and Mu or Mu + Mo selected per
+            // frequency, which is faster there (C1: 0.50 against 0.55 ms
+            // folded).
             const cplx q = cmul_fast(wt, ie);
-            const double q_im_wti = q.im * wt.im, q_re_wti = q.re * wt.im;
             // wdiff.im^2 rounded as the reference's |wdiff|^2 (:27, :33) rounds it
             const double wti2 = wt.im * wt.im;
+            // q.im wt.im and q.re wt.im: kept per element when P is formed every
+            // frequency, recomputed on the rare paths otherwise
+            const double q_im_wti_e = FOLD ? 0.0 : q.im * wt.im, q_re_wti_e = FOLD ? 0.0 : q.re * wt.im;
+            cplx mo = czero(), om2 = czero(), mum = mu, u0 = czero(), uo = czero(), w1 = czero();
+            double u0_im_wti = 0.0, u0_re_wti = 0.0, uo_im_wti = 0.0, uo_re_wti = 0.0, ie_cut = 0.0, wt2 = 0.0;
+            if constexpr (FOLD) {
+                u0 = cmul_fast(q, mu);
+                u0_im_wti = u0.im * wt.im;
+                u0_re_wti = u0.re * wt.im;
+            }
             if (nv > 0) {
                 if constexpr (NV > 0) {
 #pragma unroll
@@ -269,7 +283,14 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                                                 a.aqsn[(size_t)n1 * ncouls + ig]));
                 }
                 om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
-                mum = cadd(mu, mo);           // sch = P (branch 1) for the occupied bands
+                if constexpr (FOLD) {
+                    uo = cmul_fast(q, mo);
+                    uo_im_wti = uo.im * wt.im;
+                    uo_re_wti = uo.re * wt.im;
+                    w1 = cmul_fast(om2, mo);
+                } else {
+                    mum = cadd(mu, mo);
+                }
                 wt2 = __builtin_fma(wt.re, wt.re, wti2);
                 if (any_neg) {
                     // A real (uniform) branch: the empty asm keeps the compiler
@@ -283,9 +304,11 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 const double wdr = wxv[f] - wt.re;        // wdiff = (wdr, -wt.im), :27
                 const double wdiffr = wdr * wdr + wti2;   // |wdiff|^2 exactly as the reference rounds it
                 const double rden = rcp_1nr(wdiffr);
-                const cplx p = {__builtin_fma(q.re, wdr, -q_im_wti) * rden, __builtin_fma(q.im, wdr, q_re_wti) * rden};
-                cplx m = mu;
-                bool branch1 = false;
+                if constexpr (FOLD) {  // P Mu
+                    sach[f].re = __builtin_fma(__builtin_fma(u0.re, wdr, -u0_im_wti), rden, sach[f].re);
+                    sach[f].im = __builtin_fma(__builtin_fma(u0.im, wdr, u0_re_wti), rden, sach[f].im);
+                }
+                bool branch1 = false, above2 = false;
                 if (nv > 0) {
                     // The branch tests of :35 and :43 on |delw|^2.  est = |wt|^2 /
                     // |wdiff|^2 is within 1e-14 of the reference's rounded |delw|^2,
@@ -298,7 +321,8 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                     const double est = wt2 * rden;
                     const bool sure = (est < kEstLo2) | ((est > kEstHi2) & (est < kEstLo1)) |
                                       ((est > kEstHi1) & (est <= __DBL_MAX__));
-                    bool below1 = est < kEstLo1, above2 = est > kEstHi2;
+                    bool below1 = est < kEstLo1;
+                    above2 = est > kEstHi2;
                     if (!sure) {
                         const cplx wdiff = cdminus(wxv[f], wt);
                         const cplx delw = cscale(cmul(wt, cconj(wdiff)), rcp_refined(wdiffr));
@@ -306,28 +330,51 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                         below1 = delwr < limitone;
                         above2 = delwr > kTo1;
                     }
-                    cplx ssx = czero();
-                    if ((wdiffr > 0.25) && below1) {
-                        branch1 = true;
-                        m = mum;
-                        ssx = cscale(om2, c1[f]);
-                    } else if (above2) {
-                        // :46-50 with cden = 4 wt^2 (0.5 + delw), Omega2 = wt^2 I_eps
-                        // and delw = wt / (wx - wt):
-                        //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
-                        //                                     = -(I_eps wt / 2) / (wx + wt),
-                        // a pole at -wt, far from cancellation for wx > 0.
-                        const double s = wxv[f] + wt.re;
-                        const double rr = 0.5 * rcp_1nr(__builtin_fma(s, s, wti2));
-                        ssx = cplx{-__builtin_fma(q.re, s, q.im * wt.im) * rr,
-                                   -__builtin_fma(q.im, s, -(q.re * wt.im)) * rr};
-                    }
-                    if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
-                    sasx[f] = cfma(ssx, mo, sasx[f]);
+                    branch1 = (wdiffr > 0.25) && below1;
                 }
-                sach[f] = cfma(p, m, sach[f]);
+                // Branch 2 (:46-50) with cden = 4 wt^2 (0.5 + delw), Omega2 =
+                // wt^2 I_eps and delw = wt / (wx - wt):
+                //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
+                //                                     = -(q / 2) / (wx + wt),
+                // a pole at -wt, far from cancellation for wx > 0, with
+                // wx + wt = (s, wt.im).
+                if (!FOLD || wxv[f] < 0.00) {
+                    const double q_im_wti = FOLD ? q.im * wt.im : q_im_wti_e;
+                    const double q_re_wti = FOLD ? q.re * wt.im : q_re_wti_e;
+                    const cplx p = {__builtin_fma(q.re, wdr, -q_im_wti) * rden,
+                                    __builtin_fma(q.im, wdr, q_re_wti) * rden};
+                    if (nv > 0) {
+                        cplx ssx = czero();
+                        if (branch1) {
+                            ssx = cscale(om2, c1[f]);
+                        } else if (above2) {
+                            const double s = wxv[f] + wt.re;
+                            const double rr = 0.5 * rcp_1nr(__builtin_fma(s, s, wti2));
+                            ssx = cplx{-__builtin_fma(q.re, s, q_im_wti) * rr, -__builtin_fma(q.im, s, -q_re_wti) * rr};
+                        }
+                        // the ssx cutoff of :58-59 (only at negative frequencies)
+                        if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
+                        sasx[f] = cfma(ssx, mo, sasx[f]);
+                    }
+                    if constexpr (FOLD) {
+                        if (branch1) sach[f] = cfma(p, mo, sach[f]);  // P Mu is in already
+                    } else {
+                        sach[f] = cfma(p, branch1 ? mum : mu, sach[f]);
+                    }
+                } else if (branch1) {  // P Mo and c1 Omega2 Mo
+                    sach[f].re = __builtin_fma(__builtin_fma(uo.re, wdr, -uo_im_wti), rden, sach[f].re);
+                    sach[f].im = __builtin_fma(__builtin_fma(uo.im, wdr, uo_re_wti), rden, sach[f].im);
+                    sasx[f].re = __builtin_fma(c1[f], w1.re, sasx[f].re);
+                    sasx[f].im = __builtin_fma(c1[f], w1.im, sasx[f].im);
+                } else if (above2) {  // -(uo / 2) conj(wx + wt) / |wx + wt|^2
+                    const double s = wxv[f] + wt.re;
+                    const double rr = -0.5 * rcp_1nr(__builtin_fma(s, s, wti2));
+                    sasx[f].re = __builtin_fma(rr, __builtin_fma(uo.re, s, uo_im_wti), sasx[f].re);
+                    sasx[f].im = __builtin_fma(rr, __builtin_fma(uo.im, s, -uo_re_wti), sasx[f].im);
+                }
                 if (P2 && iw0 + f == a.acht_iw && iw0 + f < iw_end) {
-                    a.p2[row + ig] = p;
+                    a.p2[row + ig] = cplx{__builtin_fma(q.re, wdr, -(FOLD ? q.im * wt.im : q_im_wti_e)) * rden,
+                                          __builtin_fma(q.im, wdr, FOLD ? q.re * wt.im : q_re_wti_e) * rden};
                     if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
                 }
             }
@@ -345,9 +392,20 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
             }
         }
     };
+    // The choice depends on the problem's frequency count only, so every
+    // grouping of its frequencies into passes (and blocks) sums alike.
+    using fold_off = std::integral_constant<bool, false>;
+    using fold_on = std::integral_constant<bool, true>;
     int iw0 = iw_begin;
-    for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), iw0);
-    if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), iw0);
+    if constexpr (FC <= 3) {
+        for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), fold_off(), iw0);
+        if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_off(), iw0);
+    } else if (a.nfreq < 4) {  // at most 3 frequencies: one pass FCT (>= 4) wide
+        if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_off(), iw0);
+    } else {
+        for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), fold_on(), iw0);
+        if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_on(), iw0);
+    }
 }
 
 // acht_n1_loc (gppComplex.cpp:381, sch_array[2] of each band and column):

@@ -57,6 +57,27 @@ def test_factored_branch_thresholds(ctx, oracle, form):
     assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
 
 
+NEG_WX = {"nfreq3": [-2.0, -0.5, 3.0], "nfreq4": [-2.0, -0.5, 1.0, 3.0],
+          "nfreq5": [-3.0, -0.7, 0.4, 2.0, 4.0]}
+
+
+@pytest.mark.parametrize("wx", list(NEG_WX), ids=list(NEG_WX))
+@pytest.mark.parametrize("form", FACTORED)
+def test_factored_negative_frequencies(ctx, oracle, form, wx):
+    """Negative frequencies make the ssx cutoff of gppComplex.cpp:58-59 live
+    (|ssx| > 4 |I_eps| is dropped): per-frequency P (passes of up to 3
+    frequencies) and the products folded once per element (passes of 4 and
+    8) both take their explicit-ssx branch there."""
+    arrs = gpp_inputs.seeded_inputs(20, 1500, 60, nfreq=len(NEG_WX[wx]), seed=61)
+    arrs["wx_array"] = np.array(NEG_WX[wx])
+    # branch-1 elements at wx = -0.5 with |wt| > 1 have |ssx| = |wt|^2 |I_eps| / wx^2 > 4 |I_eps|
+    wt = arrs["wtilde_array"]
+    assert np.count_nonzero((np.abs(wt) > 1.0) & (np.abs(-0.5 - wt) > 0.5)) > 1000
+    ref = oracle.sigma(arrs, nvband=4, threads=8)
+    assert_close(run_gpu(ctx, arrs, 4, form), ref, TOL_FACTORED, KEYS)
+    assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
+
+
 @pytest.mark.parametrize("ncouls", [800, 3200])
 @pytest.mark.parametrize("form", FACTORED)
 def test_factored_readme_golden(ctx, oracle, ncouls, form):
