@@ -3,15 +3,14 @@
 //
 // Each stage of the factored form (gpp_factored_dev.hpp) becomes ONE
 // persistent launch over a work list that spans every instance:
-//   gather    flat element index over all instances' Bg[nb][ngpown]
-//   achstemp  (instance, band) items
+//   achstemp  (instance, band) items, one wave each, also gathering Bg
 //   Gram      (instance, G' tile, G tile) items of the FP64-MFMA product
-//   reduce    (instance, column, G-vector split, frequency chunk) items
+//   reduce    (instance, column, G-vector split) items, every frequency
 //   final     one block per instance
 // Lists are built on the host, sorted by decreasing cost, and walked with a
 // static round-robin (block b takes items b, b+G, b+2G, ...), so every item is
-// computed by exactly one block and results do not depend on scheduling.
-// The five launches are captured once into a hipGraph and replayed.
+// computed by exactly one block (or wave) and results do not depend on
+// scheduling.  The launches are captured once into a hipGraph and replayed.
 //
 // The Ozaki kinds replace the Gram launch by three: column exponents and
 // digit planes of every instance's Bg and aqsn (gpp_ozaki_dev.hpp), then the
@@ -40,36 +39,16 @@ struct BInst {
     int oz_ngp_pad, oz_nc_pad, oz_kpad;
 };
 
-__device__ __forceinline__ int find_inst(const size_t *off, int n, size_t t)
-{
-    int lo = 0, hi = n - 1;  // largest i with off[i] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (off[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__global__ void k_batch_gather(const BInst *__restrict__ insts, int n, const size_t *__restrict__ off)
-{
-    const size_t total = off[n];
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = find_inst(off, n, t);
-        const BInst &b = insts[i];
-        gather_elem(b.aqsm, b.idx, b.ncouls, b.ngpown, t - off[i], b.bg);
-    }
-}
-
+// One wave per (instance, band) item.
 __global__ __launch_bounds__(kRedThreads) void k_batch_achstemp(const BInst *__restrict__ insts,
                                                                 const int2 *__restrict__ items, int nitems)
 {
-    __shared__ cplx scratch[kRedThreads / kWave];
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    constexpr int W = kRedThreads / kWave;
+    for (int w = blockIdx.x * W + threadIdx.x / kWave; w < nitems; w += gridDim.x * W) {
         const int2 it = items[w];
         const BInst &b = insts[it.x];
-        const cplx s = achstemp_band(b.aqsm, b.aqsn, b.ieps, b.idx, b.vcoul, b.ncouls, b.ngpown, it.y, scratch);
-        if (threadIdx.x == 0) b.part_achs[it.y] = s;
+        const cplx s = achstemp_band(b.aqsm, b.aqsn, b.ieps, b.idx, b.vcoul, b.ncouls, b.ngpown, it.y, b.bg);
+        if (threadIdx.x % kWave == 0) b.part_achs[it.y] = s;
     }
 }
 
@@ -229,7 +208,6 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
     }
     // device tables after the per-instance areas
     pl.tab_inst = off;  off += align256(sizeof(BInst) * pl.n);
-    pl.tab_goff = off;  off += align256(sizeof(size_t) * (pl.n + 1));
     // work lists
     std::vector<std::pair<long, GramItem>> g;
     std::vector<std::pair<long, RedItem>> r[kRedClasses];
@@ -302,7 +280,6 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
 {
     char *w = static_cast<char *>(work);
     std::vector<BInst> tab(pl.n);
-    std::vector<size_t> goff(pl.n + 1, 0);
     for (int i = 0; i < pl.n; ++i) {
         const DevProblem &p = probs[i];
         const BatchPlan::InstOff &o = pl.inst_off[i];
@@ -330,12 +307,9 @@ hipError_t upload_batch(const BatchPlan &pl, const std::vector<DevProblem> &prob
             b.oz_nc_pad = L.nc_pad;
             b.oz_kpad = L.kpad;
         }
-        goff[i + 1] = goff[i] + (size_t)p.nb * p.ngpown;
     }
     hipError_t e;
     if ((e = hipMemcpyAsync(w + pl.tab_inst, tab.data(), sizeof(BInst) * pl.n, hipMemcpyHostToDevice, s))) return e;
-    if ((e = hipMemcpyAsync(w + pl.tab_goff, goff.data(), sizeof(size_t) * (pl.n + 1), hipMemcpyHostToDevice, s)))
-        return e;
     if (!pl.gram_items.empty() &&
         (e = hipMemcpyAsync(w + pl.tab_gram, pl.gram_items.data(), sizeof(GramItem) * pl.gram_items.size(),
                             hipMemcpyHostToDevice, s)))
@@ -408,10 +382,8 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
 {
     char *w = static_cast<char *>(work);
     const BInst *insts = reinterpret_cast<const BInst *>(w + pl.tab_inst);
-    const size_t *goff = reinterpret_cast<const size_t *>(w + pl.tab_goff);
     const int G = persistent_blocks;
     if (which & 1) {
-        hipLaunchKernelGGL(k_batch_gather, dim3(G), dim3(256), 0, s, insts, pl.n, goff);
         hipLaunchKernelGGL(k_batch_achstemp, dim3(std::min<size_t>(G, pl.ach_items.size() + 1)), dim3(kRedThreads), 0,
                            s, insts, reinterpret_cast<const int2 *>(w + pl.tab_ach), (int)pl.ach_items.size());
     }

@@ -171,12 +171,15 @@ __device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rw
     }
 }
 
-// reduce_achstemp (gppComplex.cpp:66-101) for band n1, summed over the
-// columns in a fixed block order (block_sum; every thread gets the total).
+// reduce_achstemp (gppComplex.cpp:66-101) for band n1 by one wave: lane l
+// sums columns l, l + 64, ... in order, then the xor butterfly (every lane
+// gets the same, bitwise reproducible total).  With bg non-null it also
+// stores the gathered row Bg[n1][j] = aqsm[n1][igp(j)] the factored forms
+// use: the scattered column reads of aqsm are shared by both.
 // The else branch of :94-98 is unreachable once negative indices are refused.
 __device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
                                                const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
-                                               int n1, cplx *scratch);
+                                               int n1, cplx *bg);
 
 // igp = min(inv_igp_index[j], ncouls-1), gppComplex.cpp:342-344.  Negative
 // entries are rejected before any launch; the max(.,0) only keeps a kernel
@@ -190,12 +193,14 @@ __device__ __forceinline__ int igp_of(const int32_t *idx, int j, int ncouls)
 
 __device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
                                                const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
-                                               int n1, cplx *scratch)
+                                               int n1, cplx *bg)
 {
     cplx s = czero();
-    for (int j = threadIdx.x; j < ngpown; j += blockDim.x) {
+    for (int j = threadIdx.x % kWave; j < ngpown; j += kWave) {
         const int igp = igp_of(idx, j, ncouls);
-        const cplx mygpvar1 = cconj(aqsm[(size_t)n1 * ncouls + igp]);           // :84
+        const cplx m = aqsm[(size_t)n1 * ncouls + igp];
+        if (bg) bg[(size_t)n1 * ngpown + j] = m;
+        const cplx mygpvar1 = cconj(m);                                          // :84
         const cplx ie = ieps[(size_t)j * ncouls + igp];
         const cplx schs = cplx{-ie.re, -ie.im};                                  // :86-88
         const cplx matngmatmgp = cmul(aqsn[(size_t)n1 * ncouls + igp], mygpvar1); // :89
@@ -203,7 +208,7 @@ __device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn
         if (cabs_(schs) > kTo1) cacc(schstemp, cmul(matngmatmgp, schs));          // :91-92
         cacc(s, cscale(schstemp, vcoul[igp] * 0.5));                              // :99
     }
-    return block_sum(s, scratch);
+    return cplx{wave_sum(s.re), wave_sum(s.im)};
 }
 
 }  // namespace gpp

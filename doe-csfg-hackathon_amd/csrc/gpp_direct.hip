@@ -45,6 +45,7 @@ struct DirectArgs {
     cplx *part_achs;  // [nb]
     cplx *band_ach;   // [nb][nfreq]
     cplx *band_asx;   // [max(nv,1)][nfreq]
+    cplx *bg;         // optional gathered rows of aqsm (factored forms), else null
 };
 
 struct WorkLayout {
@@ -83,6 +84,7 @@ DirectArgs make_args(const DevProblem &p, void *work)
     a.part_achs = reinterpret_cast<cplx *>(w + L.part_achs);
     a.band_ach = reinterpret_cast<cplx *>(w + L.band_ach);
     a.band_asx = reinterpret_cast<cplx *>(w + L.band_asx);
+    a.bg = nullptr;
     return a;
 }
 
@@ -328,14 +330,15 @@ __global__ __launch_bounds__(kOccWaves *kWave) void k_occ(DirectArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// achstemp per band, reduce_achstemp (gppComplex.cpp:66-101).  Grid nb,
-// block kReduceThreads; fixed-order block tree over j.
+// achstemp per band, reduce_achstemp (gppComplex.cpp:66-101): one wave per
+// band, kAchWaves bands per block.
+constexpr int kAchWaves = kReduceThreads / kWave;
 __global__ __launch_bounds__(kReduceThreads) void k_achstemp(DirectArgs a)
 {
-    __shared__ cplx scratch[kReduceThreads / kWave];
-    const int n1 = blockIdx.x;
-    const cplx s = achstemp_band(a.aqsm, a.aqsn, a.ieps, a.idx, a.vcoul, a.ncouls, a.ngpown, n1, scratch);
-    if (threadIdx.x == 0) a.part_achs[n1] = s;
+    const int n1 = blockIdx.x * kAchWaves + threadIdx.x / kWave;
+    if (n1 >= a.nb) return;
+    const cplx s = achstemp_band(a.aqsm, a.aqsn, a.ieps, a.idx, a.vcoul, a.ncouls, a.ngpown, n1, a.bg);
+    if (threadIdx.x % kWave == 0) a.part_achs[n1] = s;
 }
 
 // Per band: partials over column tiles -> band_ach[n1][iw] (and band_asx
@@ -483,7 +486,7 @@ hipError_t run_direct(const DevProblem &p, bool strict, void *work, const DevRes
     GPP_TRY(launch_unocc(a, strict, s));
     if (timer) timer->mark(s, strict ? "direct_unoccupied_strict" : "direct_unoccupied");
     if (p.nb > 0) {
-        hipLaunchKernelGGL(k_achstemp, dim3(p.nb), dim3(kReduceThreads), 0, s, a);
+        hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
         GPP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_reduce_bands, dim3(p.nb), dim3(kReduceThreads), 0, s, a, 0);
         GPP_TRY(hipGetLastError());
@@ -514,7 +517,7 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, p.nv, p.nb, 0);
     } else {
         if (p.nb > 0) {
-            hipLaunchKernelGGL(k_achstemp, dim3(p.nb), dim3(kReduceThreads), 0, s, a);
+            hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
             GPP_TRY(hipGetLastError());
         }
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, 0, 0, 1);
@@ -522,12 +525,13 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
     return hipGetLastError();
 }
 
-hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, hipStream_t s)
+hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, cplx *bg, hipStream_t s)
 {
     if (p.nb <= 0) return hipSuccess;
     DirectArgs a = make_args(p, nullptr);
     a.part_achs = part_achs;
-    hipLaunchKernelGGL(k_achstemp, dim3(p.nb), dim3(kReduceThreads), 0, s, a);
+    a.bg = bg;
+    hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
     return hipGetLastError();
 }
 

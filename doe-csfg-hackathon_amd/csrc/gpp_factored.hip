@@ -2,7 +2,8 @@
 // one problem per call (the batched persistent form is gpp_batch.hip).
 //
 // Stages (device bodies in gpp_factored_dev.hpp):
-//   1. k_gather_aqsm   Bg[n1][j] = aqsm[n1][igp(j)]                      (13 MB at 512x1638)
+//   1. k_achstemp      Bg[n1][j] = aqsm[n1][igp(j)] (13 MB at 512x1638) and the
+//                      achstemp band sums (reduce_achstemp, gppComplex.cpp:66-101)
 //   2. Gram product    Mu = conj(Bg)^T * aqsn over bands [nv, nb):
 //        FP64 MFMA (k_gram_f64, tile chosen by GramKind::tile), or the Ozaki
 //        split of gpp_ozaki.hip; written to HBM as [ngpown][ncouls]
@@ -81,14 +82,6 @@ FactLayout fact_layout(const DevProblem &p, const GramKind &k)
     }
     L.total = off;
     return L;
-}
-
-__global__ void k_gather_aqsm(const cplx *__restrict__ aqsm, const int32_t *__restrict__ idx, int nb, int ncouls,
-                              int ngpown, cplx *__restrict__ bg)
-{
-    const size_t n = (size_t)nb * ngpown;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        gather_elem(aqsm, idx, ncouls, ngpown, t, bg);
 }
 
 // 1-D grid over the ceil(ngpown/TJ) x ceil(ncouls/TI) tiles, G' tiles
@@ -291,15 +284,7 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     cplx *part_achs = reinterpret_cast<cplx *>(w + L.part_achs);
 
     if (timer) timer->begin(s);
-    {
-        const size_t n = (size_t)p.nb * p.ngpown;
-        size_t blocks = (n + 255) / 256;
-        if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(k_gather_aqsm, dim3((unsigned)blocks), dim3(256), 0, s, p.aqsm, p.idx, p.nb, p.ncouls,
-                           p.ngpown, bg);
-        GPP_TRY(hipGetLastError());
-    }
-    GPP_TRY(launch_achstemp(p, part_achs, s));
+    GPP_TRY(launch_achstemp(p, part_achs, bg, s));  // also gathers Bg
     if (timer) timer->mark(s, "factored_gather_achstemp");
     if (kind.ozaki_slices) {
         GPP_TRY(launch_gram_ozaki(p, kind.ozaki_slices, kind.int8, kind.layout, kind.group, bg, w + L.total, mu, s));

@@ -29,14 +29,6 @@ template <int WJN, int WIN> struct GramCfg {
     static constexpr int TJ = kWJ * WJN, TI = kWI * WIN, NT = 64 * WJN * WIN;
 };
 
-// Stage 1: Bg[n1][j] = aqsm[n1][igp(j)] for element t of [nb x ngpown].
-__device__ __forceinline__ void gather_elem(const cplx *aqsm, const int32_t *idx, int ncouls, int ngpown, size_t t,
-                                            cplx *bg)
-{
-    const int n1 = (int)(t / ngpown), j = (int)(t % ngpown);
-    bg[t] = aqsm[(size_t)n1 * ncouls + igp_of(idx, j, ncouls)];
-}
-
 // Stage 2: one TJ x TI tile of Mu = conj(Bg)^T * aqsn over bands [k_begin, k_end)
 // on v_mfma_f64_16x16x4f64.  Lane layout (measured on gfx950): A[l%16][l/16],
 // B[l/16][l%16], result r of lane l = C[4r + l/16][l%16]; MFMA rows are G'

@@ -87,7 +87,9 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
                            hipStream_t s);
 
 // reduce_achstemp per band into part_achs[nb] (fixed-order sums over j).
-hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, hipStream_t s);
+// achstemp per band into part_achs[nb]; with bg non-null also the gathered
+// rows Bg[n1][j] = aqsm[n1][igp(j)] [nb][ngpown].
+hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, cplx *bg, hipStream_t s);
 
 // Counts negative inv_igp_index entries on the device (result in *d_count).
 hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hipStream_t s);
@@ -172,7 +174,7 @@ struct BatchPlan {
     std::vector<int4> oz_exp_items, oz_split_items;
     std::vector<GramItem> oz_gram_items;
     bool oz_long = false;  // an INT8 instance's band range exceeds one exact chunk
-    size_t tab_inst = 0, tab_goff = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
+    size_t tab_inst = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
     size_t tab_oz_exp = 0, tab_oz_split = 0, tab_oz_gram = 0;
 };
 
