@@ -95,18 +95,20 @@ DirectArgs make_args(const DevProblem &p, void *work)
 //   a * ((wtilde * conj(wdiff) * rden) * I_eps),  rden = 1/|wdiff|^2
 // is evaluated as ((a * (wtilde * I_eps)) * conj(wdiff)) * rden, the same
 // product re-associated; wdiff = wx - wtilde and the reciprocal are formed
-// per update as in gppComplex.cpp:139-143.
+// per update as in gppComplex.cpp:139-143, the reciprocal with one Newton
+// step (about 2^-45 relative, one-signed: a sum moves by at most ~1.4e-14).
+// wtie = wtilde * I_eps, formed once per (column, G-vector) when staged.
 template <int FC>
 __device__ __forceinline__ void unocc_update_fast(cplx (&acc)[FC], cplx cm, cplx an, cplx wt,
-                                                  cplx ie, const double (&wxv)[FC])
+                                                  cplx wtie, const double (&wxv)[FC])
 {
     const cplx a = cmul_fast(cm, an);
-    const cplx q = cmul_fast(a, cmul_fast(wt, ie));
+    const cplx q = cmul_fast(a, wtie);
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
         const double wdr = wxv[f] - wt.re;   // wdiff = (wx - wt.re, -wt.im)
         const double d = wdr * wdr + wt.im * wt.im;
-        const double r = rcp_refined(d);
+        const double r = rcp_1nr(d);
         // q * conj(wdiff) = q * (wdr, wt.im)
         const double tr = q.re * wdr - q.im * wt.im;
         const double ti = q.im * wdr + q.re * wt.im;
@@ -139,11 +141,10 @@ __device__ __forceinline__ void unocc_update_strict(cplx (&acc)[FC], cplx cm, cp
 template <int FC, bool STRICT>
 __global__ __launch_bounds__(kNW *kWave) void k_unocc(DirectArgs a)
 {
-    constexpr int TILE = kBJ * kWave;                  // elements per array per chunk
-    constexpr int PER_T = (2 * TILE) / (kNW * kWave);  // staged elements per thread
-    static_assert(PER_T >= 1 && (2 * TILE) % (kNW * kWave) == 0, "tile must split evenly over the block");
+    constexpr int TILE = kBJ * kWave;  // (column, G-vector) positions per chunk
+    static_assert(TILE <= kNW * kWave, "one position per staging thread");
     __shared__ cplx s_wt[2][kBJ][kWave];
-    __shared__ cplx s_ie[2][kBJ][kWave];
+    __shared__ cplx s_ie[2][kBJ][kWave];  // I_eps (strict) or wtilde * I_eps (fast)
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int lane = threadIdx.x % kWave;
@@ -173,32 +174,24 @@ __global__ __launch_bounds__(kNW *kWave) void k_unocc(DirectArgs a)
 #pragma unroll
         for (int f = 0; f < FC; ++f) acc[jj][f] = czero();
 
-    // Staging of one chunk's wtilde/I_eps tile.  Out-of-range entries get
-    // wtilde = i (so |wdiff|^2 >= 1) and I_eps = 0: their terms are exact zeros.
-    cplx stage[PER_T];
+    // Staging of one chunk's wtilde/I_eps tile: thread t < TILE loads both at
+    // position t; the fast form stores wtilde * I_eps, formed here once
+    // instead of once per band.  Out-of-range entries get wtilde = i (so
+    // |wdiff|^2 >= 1) and I_eps = 0: their terms are exact zeros.
+    const bool stager = threadIdx.x < TILE;
+    const int s_jj = (threadIdx.x % TILE) / kWave, s_l = threadIdx.x % kWave;
+    cplx st_wt = czero(), st_ie = czero();
     auto load_chunk = [&](int c) {
-#pragma unroll
-        for (int k = 0; k < PER_T; ++k) {
-            const int e = threadIdx.x + k * kNW * kWave;
-            const int arr = e / TILE;
-            const int r = e % TILE;
-            const int jj = r / kWave, l = r % kWave;
-            const int j = j0 + jj, ig = c * kWave + l;
-            const bool ok = j < a.ngpown && ig < ncouls;
-            const cplx *src = arr ? a.ieps : a.wtilde;
-            stage[k] = ok ? src[(size_t)j * ncouls + ig] : (arr ? czero() : mk(0.0, 1.0));
-        }
+        if (!stager) return;
+        const int j = j0 + s_jj, ig = c * kWave + s_l;
+        const bool ok = j < a.ngpown && ig < ncouls;
+        st_wt = ok ? a.wtilde[(size_t)j * ncouls + ig] : mk(0.0, 1.0);
+        st_ie = ok ? a.ieps[(size_t)j * ncouls + ig] : czero();
     };
     auto store_chunk = [&](int buf) {
-#pragma unroll
-        for (int k = 0; k < PER_T; ++k) {
-            const int e = threadIdx.x + k * kNW * kWave;
-            const int arr = e / TILE;
-            const int r = e % TILE;
-            const int jj = r / kWave, l = r % kWave;
-            if (arr) s_ie[buf][jj][l] = stage[k];
-            else s_wt[buf][jj][l] = stage[k];
-        }
+        if (!stager) return;
+        s_wt[buf][s_jj][s_l] = st_wt;
+        s_ie[buf][s_jj][s_l] = STRICT ? st_ie : cmul_fast(st_wt, st_ie);
     };
 
     const int nchunks = (ncouls + kWave - 1) / kWave;
