@@ -196,6 +196,9 @@ __device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn
                                                int n1, cplx *bg)
 {
     cplx s = czero();
+    // unrolled: four columns' dependent loads (index, then the scattered
+    // aqsm / aqsn / I_eps / vcoul elements) in flight per lane
+#pragma unroll 4
     for (int j = threadIdx.x % kWave; j < ngpown; j += kWave) {
         const int igp = igp_of(idx, j, ncouls);
         const cplx m = aqsm[(size_t)n1 * ncouls + igp];

@@ -324,9 +324,9 @@ __global__ __launch_bounds__(kOccWaves *kWave) void k_occ(DirectArgs a)
 
 // ---------------------------------------------------------------------------
 // achstemp per band, reduce_achstemp (gppComplex.cpp:66-101): one wave per
-// band, kAchWaves bands per block.
-constexpr int kAchWaves = kReduceThreads / kWave;
-__global__ __launch_bounds__(kReduceThreads) void k_achstemp(DirectArgs a)
+// band and block, so that the few hundred bands spread over every CU.
+constexpr int kAchWaves = 1;
+__global__ __launch_bounds__(kAchWaves * kWave) void k_achstemp(DirectArgs a)
 {
     const int n1 = blockIdx.x * kAchWaves + threadIdx.x / kWave;
     if (n1 >= a.nb) return;
@@ -479,7 +479,7 @@ hipError_t run_direct(const DevProblem &p, bool strict, void *work, const DevRes
     GPP_TRY(launch_unocc(a, strict, s));
     if (timer) timer->mark(s, strict ? "direct_unoccupied_strict" : "direct_unoccupied");
     if (p.nb > 0) {
-        hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
+        hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
         GPP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_reduce_bands, dim3(p.nb), dim3(kReduceThreads), 0, s, a, 0);
         GPP_TRY(hipGetLastError());
@@ -510,7 +510,7 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, p.nv, p.nb, 0);
     } else {
         if (p.nb > 0) {
-            hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
+            hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
             GPP_TRY(hipGetLastError());
         }
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, 0, 0, 1);
@@ -524,7 +524,7 @@ hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, cplx *bg, hipSt
     DirectArgs a = make_args(p, nullptr);
     a.part_achs = part_achs;
     a.bg = bg;
-    hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kReduceThreads), 0, s, a);
+    hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
     return hipGetLastError();
 }
 
