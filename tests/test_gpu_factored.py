@@ -234,6 +234,38 @@ def test_ozaki_option_validation(ctx):
 ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 
+@pytest.mark.parametrize("nfreq", [4, 11, 13])
+def test_reduce_schedules_agree(ctx, oracle, nfreq):
+    """Frequencies per pass (4 or 8, the last pass 4 wide when at most 4
+    remain) and per block (all, 4, 8) regroup the same per-frequency sums:
+    bit-identical for one G-range size; every schedule, at G ranges of 256
+    and 8192, within 1e-10 of the oracle."""
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(30, 3000, 64, nfreq=nfreq, seed=70 + nfreq, dw=0.3)
+    ctx.upload(problem_of(arrs, 4), arrs)
+    ref = oracle.sigma(arrs, nvband=4, threads=8)
+    try:
+        for chunk in (256, 8192):
+            ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, chunk)
+            base = None
+            for fc in (4, 8):
+                for fpb in (0, 4, 8):
+                    ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, fc)
+                    ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, fpb)
+                    out = ctx.sigma("factored-f64")
+                    assert_close(out, ref, TOL_FACTORED, KEYS)
+                    if base is None:
+                        base = out
+                    for k in KEYS:
+                        np.testing.assert_array_equal(out[k], base[k], err_msg=f"{chunk} {fc} {fpb} {k}")
+        ach = ctx.sigma("factored-f64", acht_n1_loc=True)
+        assert_close(ach, ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, 8192)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 8)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, 0)
+
+
 GRAM_TILE_SHAPES = [
     (37, 2, 12000, 300, 3),   # K = 35: a partial last band step
     (6, 2, 40000, 200, 3),    # 4 bands, fewer than one band step
