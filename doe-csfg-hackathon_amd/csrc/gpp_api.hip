@@ -970,6 +970,8 @@ struct gpp_batch {
     hipEvent_t evs[3] = {nullptr, nullptr, nullptr};
     double last_ms = 0.0;
     double stage_ms[4] = {0, 0, 0, 0};
+    cplx *host_out = nullptr;  // pinned copy of the results table
+    size_t host_out_bytes = 0;
 };
 
 namespace {
@@ -1008,6 +1010,14 @@ gpp_status batch_prepare(gpp_batch *b, const gpp::GramKind &k)
     b->plan = gpp::plan_batch(b->dps, k, &need);
     HIP_OR_FAIL(b->work.ensure(need), "batch workspace allocation");
     HIP_OR_FAIL(gpp::upload_batch(b->plan, b->dps, b->work.ptr, b->ctx->stream), "batch tables upload");
+    if (b->plan.out_bytes > b->host_out_bytes) {
+        if (b->host_out) (void)hipHostFree(b->host_out);
+        b->host_out = nullptr;
+        b->host_out_bytes = 0;
+        HIP_OR_FAIL(hipHostMalloc(reinterpret_cast<void **>(&b->host_out), b->plan.out_bytes, hipHostMallocDefault),
+                    "pinned results allocation");
+        b->host_out_bytes = b->plan.out_bytes;
+    }
     b->kind = k;
     b->planned = true;
     return GPP_OK;
@@ -1182,14 +1192,9 @@ gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
         }
     }
     HIP_OR_FAIL(hipEventRecord(b->ev1, s), "hipEventRecord");
-    std::vector<std::vector<cplx>> host(b->n);
-    for (int i = 0; i < b->n; ++i) {
-        const gpp::DevProblem &dp = b->dps[i];
-        host[i].resize((size_t)2 * dp.nfreq + dp.nb + 1);
-        const char *src = static_cast<const char *>(b->work.ptr) + gpp::batch_results_offset(b->plan, i);
-        HIP_OR_FAIL(hipMemcpyAsync(host[i].data(), src, sizeof(cplx) * host[i].size(), hipMemcpyDeviceToHost, s),
-                    "D2H batch results");
-    }
+    HIP_OR_FAIL(hipMemcpyAsync(b->host_out, static_cast<const char *>(b->work.ptr) + b->plan.tab_out,
+                               b->plan.out_bytes, hipMemcpyDeviceToHost, s),
+                "D2H batch results");
     HIP_OR_FAIL(hipStreamSynchronize(s), "batch synchronize");
     float ms = 0.f;
     HIP_OR_FAIL(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
@@ -1204,9 +1209,11 @@ gpp_status gpp_batch_sigma(gpp_batch *b, gpp_form form, const gpp_outputs *outs)
     }
     for (int i = 0; i < b->n; ++i) {
         const gpp::DevProblem &dp = b->dps[i];
-        memcpy(outs[i].achtemp, host[i].data(), sizeof(cplx) * dp.nfreq);
-        memcpy(outs[i].asxtemp, host[i].data() + dp.nfreq, sizeof(cplx) * dp.nfreq);
-        if (outs[i].achstemp) memcpy(outs[i].achstemp, host[i].data() + 2 * dp.nfreq + dp.nb, sizeof(cplx));
+        const cplx *r = reinterpret_cast<const cplx *>(reinterpret_cast<const char *>(b->host_out) +
+                                                       (gpp::batch_results_offset(b->plan, i) - b->plan.tab_out));
+        memcpy(outs[i].achtemp, r, sizeof(cplx) * dp.nfreq);
+        memcpy(outs[i].asxtemp, r + dp.nfreq, sizeof(cplx) * dp.nfreq);
+        if (outs[i].achstemp) memcpy(outs[i].achstemp, r + 2 * dp.nfreq + dp.nb, sizeof(cplx));
     }
     return GPP_OK;
 }
@@ -1292,6 +1299,7 @@ gpp_status gpp_batch_destroy(gpp_batch *b)
     batch_drop_graph(b);
     for (DevBuf &d : b->owned) d.release();
     b->work.release();
+    if (b->host_out) (void)hipHostFree(b->host_out);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->evs)

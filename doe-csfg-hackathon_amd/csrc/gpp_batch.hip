@@ -194,7 +194,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         o.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
         o.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
         o.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
-        o.out = off;       off += align256(sizeof(cplx) * ((size_t)2 * p.nfreq + p.nb + 1));
+        o.out = 0;  // in the results table below
         o.oz_eb = o.oz_ea = o.oz_bd = o.oz_ad = 0;
         if (k.ozaki_slices) {
             const oz::OzLayout L = oz::oz_layout(p, k.ozaki_slices, k.int8);
@@ -206,6 +206,15 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         }
         pl.inst_off[i] = o;
     }
+    // every instance's results (achtemp, asxtemp, per-band terms, achstemp)
+    // back to back, so one copy brings them all to the host
+    pl.tab_out = off;
+    for (int i = 0; i < pl.n; ++i) {
+        pl.inst_off[i].out = off;
+        off += sizeof(cplx) * ((size_t)2 * probs[i].nfreq + probs[i].nb + 1);
+    }
+    pl.out_bytes = off - pl.tab_out;
+    off = align256(off);
     // device tables after the per-instance areas
     pl.tab_inst = off;  off += align256(sizeof(BInst) * pl.n);
     // work lists

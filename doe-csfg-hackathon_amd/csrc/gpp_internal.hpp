@@ -174,6 +174,7 @@ struct BatchPlan {
     std::vector<int4> oz_exp_items, oz_split_items;
     std::vector<GramItem> oz_gram_items;
     bool oz_long = false;  // an INT8 instance's band range exceeds one exact chunk
+    size_t tab_out = 0, out_bytes = 0;  // the results table, [2 nfreq + nb + 1] per instance
     size_t tab_inst = 0, tab_gram = 0, tab_red = 0, tab_ach = 0, total = 0;
     size_t tab_oz_exp = 0, tab_oz_split = 0, tab_oz_gram = 0;
 };
