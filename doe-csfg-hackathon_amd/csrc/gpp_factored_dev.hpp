@@ -217,29 +217,20 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 
         const size_t row = (size_t)j * ncouls;
         // The next element's wtilde, I_eps and Mu load while this one is
-        // evaluated (two elements in flight per thread).
-        constexpr bool PREFETCH = true;
+        // evaluated (two elements in flight per thread; a third did not
+        // help).
         cplx wt_n = czero(), ie_n = czero(), mu_n = czero();
-        if (PREFETCH && ig_begin + (int)threadIdx.x < ig_end) {
+        if (ig_begin + (int)threadIdx.x < ig_end) {
             wt_n = a.wtilde[row + ig_begin + threadIdx.x];
             ie_n = a.ieps[row + ig_begin + threadIdx.x];
             mu_n = a.mu[row + ig_begin + threadIdx.x];
         }
         for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
-            cplx wt, ie, mu;
-            if constexpr (PREFETCH) {
-                wt = wt_n;
-                ie = ie_n;
-                mu = mu_n;
-                if (ig + (int)blockDim.x < ig_end) {
-                    wt_n = a.wtilde[row + ig + blockDim.x];
-                    ie_n = a.ieps[row + ig + blockDim.x];
-                    mu_n = a.mu[row + ig + blockDim.x];
-                }
-            } else {
-                wt = a.wtilde[row + ig];
-                ie = a.ieps[row + ig];
-                mu = a.mu[row + ig];
+            const cplx wt = wt_n, ie = ie_n, mu = mu_n;
+            if (ig + (int)blockDim.x < ig_end) {
+                wt_n = a.wtilde[row + ig + blockDim.x];
+                ie_n = a.ieps[row + ig + blockDim.x];
+                mu_n = a.mu[row + ig + blockDim.x];
             }
             // P = delw I_eps = q conj(wdiff) / |wdiff|^2 with q = wt I_eps and
             // conj(wdiff) = (wx - wt.re, wt.im) =: (wdr, wt.im).  FOLD (problems
