@@ -347,7 +347,9 @@ void launch_batch_oz_s(int G, hipStream_t s, const BInst *insts, const int4 *si,
                        bool long_range)
 {
     constexpr int NBUF = oz::oz3_buffers(S);
-    hipLaunchKernelGGL((k_batch_oz_split<S, D>), dim3(std::max(1, std::min(G, ns))), dim3(256), 0, s, insts, si, ns);
+    // the split streams memory: up to eight blocks per CU in flight (G is two)
+    hipLaunchKernelGGL((k_batch_oz_split<S, D>), dim3(std::max(1, std::min(4 * G, ns))), dim3(256), 0, s, insts, si,
+                       ns);
     // one 256-thread block per CU: G is two blocks per CU
     const dim3 grid(std::max(1, std::min((G + 1) / 2, ng)));
     if (long_range)
@@ -402,7 +404,8 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         const GramItem *gi = reinterpret_cast<const GramItem *>(w + pl.tab_oz_gram);
         const int ne = (int)pl.oz_exp_items.size(), ns = (int)pl.oz_split_items.size();
         const int ng = (int)pl.oz_gram_items.size();
-        hipLaunchKernelGGL(k_batch_oz_colexp, dim3(std::max(1, std::min(G, ne))), dim3(256), 0, s, insts, ei, ne);
+        hipLaunchKernelGGL(k_batch_oz_colexp, dim3(std::max(1, std::min(4 * G, ne))), dim3(256), 0, s, insts, ei,
+                           ne);
         if (pl.kind.int8)
             launch_batch_oz<int8_t>(pl.kind.ozaki_slices, G, s, insts, si, ns, gi, ng, pl.oz_long);
         else
