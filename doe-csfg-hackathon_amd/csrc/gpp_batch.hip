@@ -39,15 +39,26 @@ struct BInst {
     int oz_ngp_pad, oz_nc_pad, oz_kpad;
 };
 
-// One wave per (instance, band) item.
+// The columns' achstemp entries: grid (column blocks of 256, instance).
+__global__ __launch_bounds__(256) void k_batch_ach_cols(const BInst *__restrict__ insts)
+{
+    const BInst &b = insts[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < b.ngpown)
+        ach_col(b.ieps, b.idx, b.vcoul, b.ncouls, j, reinterpret_cast<AchCol *>(b.part_achs + ach_cols_entry(b.nb)));
+}
+
+// One wave per (instance, band, column group) item.
 __global__ __launch_bounds__(kRedThreads) void k_batch_achstemp(const BInst *__restrict__ insts,
                                                                 const int2 *__restrict__ items, int nitems)
 {
     constexpr int W = kRedThreads / kWave;
     for (int w = blockIdx.x * W + threadIdx.x / kWave; w < nitems; w += gridDim.x * W) {
         const int2 it = items[w];
-        const BInst &b = insts[it.x];
-        const cplx s = achstemp_band(b.aqsm, b.aqsn, b.ieps, b.idx, b.vcoul, b.ncouls, b.ngpown, it.y, b.bg);
+        const BInst &b = insts[it.x];  // it.y = band * kAchSplit + column group
+        const AchCol *cols = reinterpret_cast<const AchCol *>(b.part_achs + ach_cols_entry(b.nb));
+        const cplx s = achstemp_band(b.aqsm, b.aqsn, cols, b.ncouls, b.ngpown, it.y / kAchSplit, it.y % kAchSplit,
+                                     b.bg);
         if (threadIdx.x % kWave == 0) b.part_achs[it.y] = s;
     }
 }
@@ -180,7 +191,10 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
 {
     BatchPlan pl;
     pl.n = (int)probs.size();
-    for (const DevProblem &p : probs) pl.max_nfreq = std::max(pl.max_nfreq, p.nfreq);
+    for (const DevProblem &p : probs) {
+        pl.max_nfreq = std::max(pl.max_nfreq, p.nfreq);
+        pl.max_ngpown = std::max(pl.max_ngpown, p.ngpown);
+    }
     pl.kind = k;
     size_t off = 0;
     pl.inst_off.resize(pl.n);
@@ -193,7 +207,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         o.mu = off;        off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
         o.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
         o.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * nsplit * p.nfreq);
-        o.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
+        o.part_achs = off; off += align256(sizeof(cplx) * ach_region_entries(p.nb, p.ngpown));
         o.out = 0;  // in the results table below
         o.oz_eb = o.oz_ea = o.oz_bd = o.oz_ad = 0;
         if (k.ozaki_slices) {
@@ -238,7 +252,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
                 const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
                 r[red_class(p.nv)].push_back({len * slots * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp}});
             }
-        for (int n1 = 0; n1 < p.nb; ++n1) a.push_back(int2{i, n1});
+        for (int t = 0; t < p.nb * kAchSplit; ++t) a.push_back(int2{i, t});
     }
     auto by_cost = [](const auto &x, const auto &y) { return x.first > y.first; };
     std::stable_sort(g.begin(), g.end(), by_cost);
@@ -395,7 +409,9 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
     const BInst *insts = reinterpret_cast<const BInst *>(w + pl.tab_inst);
     const int G = persistent_blocks;
     if (which & 1) {
-        hipLaunchKernelGGL(k_batch_achstemp, dim3(std::min<size_t>(G, pl.ach_items.size() + 1)), dim3(kRedThreads), 0,
+        if (pl.max_ngpown > 0)
+            hipLaunchKernelGGL(k_batch_ach_cols, dim3((pl.max_ngpown + 255) / 256, pl.n), dim3(256), 0, s, insts);
+        hipLaunchKernelGGL(k_batch_achstemp, dim3(std::min<size_t>(4 * G, pl.ach_items.size() + 1)), dim3(kRedThreads), 0,
                            s, insts, reinterpret_cast<const int2 *>(w + pl.tab_ach), (int)pl.ach_items.size());
     }
     if ((which & 2) && pl.kind.ozaki_slices) {

@@ -177,9 +177,32 @@ __device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rw
 // stores the gathered row Bg[n1][j] = aqsm[n1][igp(j)] the factored forms
 // use: the scattered column reads of aqsm are shared by both.
 // The else branch of :94-98 is unreachable once negative indices are refused.
-__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
-                                               const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
-                                               int n1, cplx *bg);
+// Each band's achstemp sum is split over this many waves (columns j with
+// j / 64 = q mod kAchSplit go to wave q); the partials are kept per (band, q)
+// and summed in the final stage.  One wave per band left most SIMDs idle
+// and each wave waited on a chain of dependent scattered loads.
+constexpr int kAchSplit = 4;
+
+// The band-independent part of column j's achstemp term (gppComplex.cpp:
+// 77-99): igp, schs = -I_eps[j][igp] when |schs| > to1 (else zero: the term
+// is skipped) and vcoul[igp] * 0.5, formed once per call instead of once per
+// band (the strided I_eps element and the square root were the costly part).
+struct AchCol {
+    double sre, sim, vh;
+    int igp, pad;
+};
+// The achstemp region of a workspace, in complex entries: kAchSplit partials
+// per band, then (from an even entry, 32-byte aligned) one AchCol per column.
+__host__ __device__ inline size_t ach_cols_entry(int nb) { return ((size_t)nb * kAchSplit + 2) & ~(size_t)1; }
+__host__ __device__ inline size_t ach_region_entries(int nb, int ngpown)
+{
+    return ach_cols_entry(nb) + 2 * (size_t)ngpown;
+}
+
+__device__ __forceinline__ void ach_col(const cplx *ieps, const int32_t *idx, const double *vcoul, int ncouls, int j,
+                                        AchCol *cols);
+__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const AchCol *cols, int ncouls,
+                                               int ngpown, int n1, int q, cplx *bg);
 
 // igp = min(inv_igp_index[j], ncouls-1), gppComplex.cpp:342-344.  Negative
 // entries are rejected before any launch; the max(.,0) only keeps a kernel
@@ -191,25 +214,33 @@ __device__ __forceinline__ int igp_of(const int32_t *idx, int j, int ncouls)
     return igp < 0 ? 0 : igp;
 }
 
-__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const cplx *ieps,
-                                               const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
-                                               int n1, cplx *bg)
+__device__ __forceinline__ void ach_col(const cplx *ieps, const int32_t *idx, const double *vcoul, int ncouls, int j,
+                                        AchCol *cols)
+{
+    const int igp = igp_of(idx, j, ncouls);
+    const cplx ie = ieps[(size_t)j * ncouls + igp];
+    const cplx schs = cplx{-ie.re, -ie.im};  // :86-88
+    const bool keep = cabs_(schs) > kTo1;    // :91
+    cols[j] = AchCol{keep ? schs.re : 0.0, keep ? schs.im : 0.0, vcoul[igp] * 0.5, igp, 0};
+}
+
+__device__ __forceinline__ cplx achstemp_band(const cplx *aqsm, const cplx *aqsn, const AchCol *cols, int ncouls,
+                                               int ngpown, int n1, int q, cplx *bg)
 {
     cplx s = czero();
-    // unrolled: four columns' dependent loads (index, then the scattered
-    // aqsm / aqsn / I_eps / vcoul elements) in flight per lane
+    // unrolled: four columns' dependent loads (the column entry, then the
+    // scattered aqsm / aqsn elements) in flight per lane
 #pragma unroll 4
-    for (int j = threadIdx.x % kWave; j < ngpown; j += kWave) {
-        const int igp = igp_of(idx, j, ncouls);
-        const cplx m = aqsm[(size_t)n1 * ncouls + igp];
+    for (int j = q * kWave + threadIdx.x % kWave; j < ngpown; j += kAchSplit * kWave) {
+        const AchCol c = cols[j];
+        const cplx m = aqsm[(size_t)n1 * ncouls + c.igp];
         if (bg) bg[(size_t)n1 * ngpown + j] = m;
-        const cplx mygpvar1 = cconj(m);                                          // :84
-        const cplx ie = ieps[(size_t)j * ncouls + igp];
-        const cplx schs = cplx{-ie.re, -ie.im};                                  // :86-88
-        const cplx matngmatmgp = cmul(aqsn[(size_t)n1 * ncouls + igp], mygpvar1); // :89
-        cplx schstemp = czero();
-        if (cabs_(schs) > kTo1) cacc(schstemp, cmul(matngmatmgp, schs));          // :91-92
-        cacc(s, cscale(schstemp, vcoul[igp] * 0.5));                              // :99
+        if (c.sre != 0.0 || c.sim != 0.0) {  // |schs| > to1: schs is nonzero
+            const cplx matngmatmgp = cmul(aqsn[(size_t)n1 * ncouls + c.igp], cconj(m));  // :84, :89
+            cplx schstemp = czero();
+            cacc(schstemp, cmul(matngmatmgp, cplx{c.sre, c.sim}));  // :92
+            cacc(s, cscale(schstemp, c.vh));                         // :99
+        }
     }
     return cplx{wave_sum(s.re), wave_sum(s.im)};
 }

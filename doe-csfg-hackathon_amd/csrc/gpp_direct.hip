@@ -42,7 +42,7 @@ struct DirectArgs {
     double occ;
     cplx *part_ach;   // [nb][njt][nfreq]
     cplx *part_asx;   // [max(nv,1)][njt][nfreq]
-    cplx *part_achs;  // [nb]
+    cplx *part_achs;  // [nb][kAchSplit], then the AchCol entries (ach_cols_entry)
     cplx *band_ach;   // [nb][nfreq]
     cplx *band_asx;   // [max(nv,1)][nfreq]
     cplx *bg;         // optional gathered rows of aqsm (factored forms), else null
@@ -62,7 +62,7 @@ WorkLayout layout_for(const DevProblem &p)
     size_t off = 0;
     L.part_ach = off;  off += align256(sizeof(cplx) * (size_t)p.nb * njt * p.nfreq + sizeof(cplx));
     L.part_asx = off;  off += align256(sizeof(cplx) * nvx * njt * p.nfreq);
-    L.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
+    L.part_achs = off; off += align256(sizeof(cplx) * ach_region_entries(p.nb, p.ngpown));
     L.band_ach = off;  off += align256(sizeof(cplx) * ((size_t)p.nb * p.nfreq + 1));
     L.band_asx = off;  off += align256(sizeof(cplx) * nvx * p.nfreq);
     L.total = off;
@@ -328,10 +328,28 @@ __global__ __launch_bounds__(kOccWaves *kWave) void k_occ(DirectArgs a)
 constexpr int kAchWaves = 1;
 __global__ __launch_bounds__(kAchWaves * kWave) void k_achstemp(DirectArgs a)
 {
-    const int n1 = blockIdx.x * kAchWaves + threadIdx.x / kWave;
-    if (n1 >= a.nb) return;
-    const cplx s = achstemp_band(a.aqsm, a.aqsn, a.ieps, a.idx, a.vcoul, a.ncouls, a.ngpown, n1, a.bg);
-    if (threadIdx.x % kWave == 0) a.part_achs[n1] = s;
+    const int item = blockIdx.x * kAchWaves + threadIdx.x / kWave;  // band * kAchSplit + column group
+    if (item >= a.nb * kAchSplit) return;
+    const AchCol *cols = reinterpret_cast<const AchCol *>(a.part_achs + ach_cols_entry(a.nb));
+    const cplx s = achstemp_band(a.aqsm, a.aqsn, cols, a.ncouls, a.ngpown, item / kAchSplit, item % kAchSplit, a.bg);
+    if (threadIdx.x % kWave == 0) a.part_achs[item] = s;
+}
+
+__global__ __launch_bounds__(256) void k_ach_cols(DirectArgs a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < a.ngpown)
+        ach_col(a.ieps, a.idx, a.vcoul, a.ncouls, j, reinterpret_cast<AchCol *>(a.part_achs + ach_cols_entry(a.nb)));
+}
+
+// The columns' achstemp entries, then the (band, column group) partials (and
+// the Bg gather when a.bg is set).
+hipError_t launch_ach(const DirectArgs &a, hipStream_t s)
+{
+    if (a.ngpown > 0) hipLaunchKernelGGL(k_ach_cols, dim3((a.ngpown + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_achstemp, dim3((a.nb * kAchSplit + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s,
+                       a);
+    return hipGetLastError();
 }
 
 // Per band: partials over column tiles -> band_ach[n1][iw] (and band_asx
@@ -381,7 +399,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_final(DirectArgs a, c
         out[2 * nf + n1] = (n1 >= n1_begin && n1 < n1_end) ? a.band_ach[(size_t)n1 * nf + acht_iw] : czero();
     cplx h = czero();
     if (with_achs)
-        for (int n1 = threadIdx.x; n1 < a.nb; n1 += blockDim.x) h = cadd(h, a.part_achs[n1]);
+        for (int t = threadIdx.x; t < a.nb * kAchSplit; t += blockDim.x) h = cadd(h, a.part_achs[t]);
     h = block_sum(h, scratch);
     if (threadIdx.x == 0) out[2 * nf + a.nb] = h;
 }
@@ -479,8 +497,7 @@ hipError_t run_direct(const DevProblem &p, bool strict, void *work, const DevRes
     GPP_TRY(launch_unocc(a, strict, s));
     if (timer) timer->mark(s, strict ? "direct_unoccupied_strict" : "direct_unoccupied");
     if (p.nb > 0) {
-        hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
-        GPP_TRY(hipGetLastError());
+        GPP_TRY(launch_ach(a, s));
         hipLaunchKernelGGL(k_reduce_bands, dim3(p.nb), dim3(kReduceThreads), 0, s, a, 0);
         GPP_TRY(hipGetLastError());
     }
@@ -509,10 +526,7 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
         }
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, p.nv, p.nb, 0);
     } else {
-        if (p.nb > 0) {
-            hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
-            GPP_TRY(hipGetLastError());
-        }
+        if (p.nb > 0) GPP_TRY(launch_ach(a, s));
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kReduceThreads), 0, s, a, res.base, 0, 0, 1);
     }
     return hipGetLastError();
@@ -524,8 +538,7 @@ hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, cplx *bg, hipSt
     DirectArgs a = make_args(p, nullptr);
     a.part_achs = part_achs;
     a.bg = bg;
-    hipLaunchKernelGGL(k_achstemp, dim3((p.nb + kAchWaves - 1) / kAchWaves), dim3(kAchWaves * kWave), 0, s, a);
-    return hipGetLastError();
+    return launch_ach(a, s);
 }
 
 hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hipStream_t s)

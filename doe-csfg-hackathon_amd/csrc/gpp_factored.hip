@@ -70,7 +70,7 @@ FactLayout fact_layout(const DevProblem &p, const GramKind &k)
     L.m = off;         off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
     L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
-    L.part_achs = off; off += align256(sizeof(cplx) * ((size_t)p.nb + 1));
+    L.part_achs = off; off += align256(sizeof(cplx) * ach_region_entries(p.nb, p.ngpown));
     L.p2 = L.br1 = L.acht_part = L.acht_occ = off;
     L.as = AchtSplit{0, 0, 0};
     if (k.acht) {

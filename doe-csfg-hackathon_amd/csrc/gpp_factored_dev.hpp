@@ -578,7 +578,7 @@ __device__ __forceinline__ void acht_occ_item(const cplx *__restrict__ p2, const
 // Stage 4: block `part` < nfreq forms achtemp[part] / asxtemp[part] as
 // fixed-order sums over the (column, split) partials; block part == nfreq
 // sets acht_n1_loc to NaN (the factored forms write it in a later stage when
-// it is requested) and sums the per-band achstemp partials in order.
+// it is requested) and sums the per-(band, column group) achstemp partials.
 __device__ __forceinline__ void final_block(const cplx *col_ach, const cplx *col_asx, const cplx *part_achs, int ncols,
                                             int nfreq, int nb, int nv, cplx *out, cplx *scratch, int part)
 {
@@ -600,7 +600,7 @@ __device__ __forceinline__ void final_block(const cplx *col_ach, const cplx *col
     const double nan = __builtin_nan("");
     for (int n1 = threadIdx.x; n1 < nb; n1 += blockDim.x) out[2 * nfreq + n1] = cplx{nan, nan};
     cplx h = czero();
-    for (int n1 = threadIdx.x; n1 < nb; n1 += blockDim.x) h = cadd(h, part_achs[n1]);
+    for (int t = threadIdx.x; t < nb * kAchSplit; t += blockDim.x) h = cadd(h, part_achs[t]);
     h = block_sum(h, scratch);
     if (threadIdx.x == 0) out[2 * nfreq + nb] = h;
 }

@@ -163,6 +163,7 @@ struct BatchPlan {
     int n = 0;
     int fc = 8;
     int max_nfreq = 0;
+    int max_ngpown = 0;
     GramKind kind;
     std::vector<InstOff> inst_off;
     std::vector<GramItem> gram_items;
