@@ -230,3 +230,34 @@ def test_full_c1_properties(ctx):
     c = ctx.sigma("direct")
     for k in a:
         np.testing.assert_array_equal(a[k], c[k])
+
+
+@pytest.mark.parametrize("form", FORMS_DIRECT + ["factored-f64", "factored-ozaki-i8"])
+def test_achstemp_cutoff_and_clamped_columns(ctx, oracle, form):
+    """achstemp when some columns' I_eps[j][igp(j)] lies below the cutoff
+    |schs| > to1 = 1e-6 of gppComplex.cpp:91 (those columns add nothing),
+    some just above it, one is zero, and the last columns' indices run past
+    ncouls (clamped to ncouls - 1, :342-344, so several columns share one
+    G-vector).  ngpown = 300 is not a multiple of the 256 columns that one
+    band's four column groups cover per round.  The direct forms match the
+    oracle to 1e-12 on every output; the factored forms compute achstemp
+    with the same kernel (1e-12) and the other outputs to 1e-10."""
+    nb, nv, ncouls, ngpown = 14, 3, 1200, 300
+    arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, seed=77)
+    idx = arrs["inv_igp_index"].astype(np.int64)
+    idx[-3:] = [ncouls + 5, ncouls + 40, ncouls - 1]
+    arrs["inv_igp_index"] = idx.astype(np.int32)
+    igp = np.minimum(idx, ncouls - 1)
+    eps = arrs["I_eps_array"]
+    rng = np.random.default_rng(5)
+    below, above = rng.choice(ngpown - 3, size=(2, 40), replace=False)
+    eps[below, igp[below]] = 3e-7 * (1 - 1j)    # |schs| = 4.2e-7: skipped
+    eps[above, igp[above]] = -8e-7 + 8e-7j      # |schs| = 1.13e-6: kept
+    eps[0, igp[0]] = 0.0
+    ref = oracle.sigma(arrs, nvband=nv, threads=8)
+    out = run_gpu(ctx, arrs, nv, form)
+    assert_close(out, ref, keys=("achstemp",))
+    if form.startswith("direct"):
+        assert_close(out, ref)
+    else:  # acht_n1_loc is not requested from the factored forms here
+        assert_close(out, ref, tol=1e-10, keys=("achtemp", "asxtemp"))
