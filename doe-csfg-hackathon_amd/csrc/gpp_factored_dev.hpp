@@ -96,6 +96,7 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
         if (st + 1 < nst) load_stage(k_begin + (st + 1) * kKS);
+        __builtin_amdgcn_s_setprio(1);  // the MFMA section: the wave with ready operands issues first
 #pragma unroll
         for (int kk = 0; kk < kKS; kk += 4) {
             const int kr = kk + lane / 16;
@@ -121,6 +122,7 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
                 }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         if (st + 1 < nst) store_stage(buf ^ 1);
         __syncthreads();
     }
