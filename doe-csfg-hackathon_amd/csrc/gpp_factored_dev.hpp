@@ -97,6 +97,9 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
         const int buf = st & 1;
         if (st + 1 < nst) load_stage(k_begin + (st + 1) * kKS);
         __builtin_amdgcn_s_setprio(1);  // the MFMA section: the wave with ready operands issues first
+        // scheduling hint: interleave the fragment reads from LDS with the MFMAs
+        // (strategy 1; measured ~1 % faster than the default schedule and than strategy 0)
+        __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
         for (int kk = 0; kk < kKS; kk += 4) {
             const int kr = kk + lane / 16;
