@@ -28,8 +28,16 @@
  *                                asxtemp, exact-rational brute force on tiny
  *                                shapes, invariants, and the compiled
  *                                reference harness on seeded fixtures.
- *   oracle_sigma_dd           -- agrees with oracle_sigma to rounding and
- *                                with the exact-rational brute force.
+ *   oracle_sigma_dd           -- its TwoSum / dd-sum / dd-times-double
+ *                                helpers against exact rational sums and
+ *                                products (oracle_dd_* hooks); the whole
+ *                                loop bitwise equal to the correctly rounded
+ *                                exact value on dyadic inputs whose terms are
+ *                                exact but whose left-to-right sum loses
+ *                                bits (where oracle_sigma is measurably
+ *                                off), and within 2e-16 of sum|terms| of the
+ *                                exact-rational brute force on seeded,
+ *                                near-pole and cancelling shapes.
  *   oracle_sigma_wxb          -- equal rows reproduce oracle_sigma bitwise;
  *                                distinct rows: each band's acht_n1_loc equals,
  *                                bitwise, the one-band run with that band's
@@ -385,6 +393,27 @@ static inline oc_dd oc_dd_mul_d(oc_dd a, double b)
     return r;
 }
 static inline oc_ddc oc_ddc_mul_d(oc_ddc a, double b) { oc_ddc r; r.re = oc_dd_mul_d(a.re, b); r.im = oc_dd_mul_d(a.im, b); return r; }
+
+/* Test hooks for the double-double helpers above (the tests compare them with
+ * exact rational sums and products): out = {hi, lo}. */
+void oracle_dd_sum(const double *x, long n, double *out)
+{
+    oc_dd a = {0.0, 0.0};
+    for (long i = 0; i < n; ++i) a = oc_dd_add(a, x[i]);
+    out[0] = a.hi; out[1] = a.lo;
+}
+void oracle_dd_sum_dd(const double *hilo, long n, double *out)
+{
+    oc_dd a = {0.0, 0.0};
+    for (long i = 0; i < n; ++i) { oc_dd b = {hilo[2 * i], hilo[2 * i + 1]}; a = oc_dd_add_dd(a, b); }
+    out[0] = a.hi; out[1] = a.lo;
+}
+void oracle_dd_mul_d(double hi, double lo, double b, double *out)
+{
+    oc_dd a = {hi, lo};
+    oc_dd r = oc_dd_mul_d(a, b);
+    out[0] = r.hi; out[1] = r.lo;
+}
 
 int oracle_sigma_dd(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
                     const int *inv_igp_index, const double *wx_array, const double *vcoul,

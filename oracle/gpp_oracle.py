@@ -47,6 +47,12 @@ def load_oracle() -> ctypes.CDLL:
         lib.oracle_pole_terms.argtypes = [ctypes.c_double, vp, vp, vp, vp]
         lib.oracle_pole_terms.restype = ctypes.c_int
         lib.oracle_max_threads.restype = ctypes.c_int
+        lib.oracle_dd_sum.argtypes = [vp, ctypes.c_long, vp]
+        lib.oracle_dd_sum.restype = None
+        lib.oracle_dd_sum_dd.argtypes = [vp, ctypes.c_long, vp]
+        lib.oracle_dd_sum_dd.restype = None
+        lib.oracle_dd_mul_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
+        lib.oracle_dd_mul_d.restype = None
         _oracle = lib
     return _oracle
 
@@ -141,6 +147,29 @@ def pole_terms(wxt: float, wtilde: complex, ieps: complex, use_ref: bool = False
         br = load_oracle().oracle_pole_terms(wxt, wt.ctypes.data, ie.ctypes.data, sch.ctypes.data,
                                              ssx.ctypes.data)
     return br, complex(sch[0], sch[1]), complex(ssx[0], ssx[1])
+
+
+def dd_sum(x) -> tuple:
+    """(hi, lo) of the oracle's double-double running sum of x, in order."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(2)
+    load_oracle().oracle_dd_sum(x.ctypes.data, x.size, out.ctypes.data)
+    return float(out[0]), float(out[1])
+
+
+def dd_sum_dd(pairs) -> tuple:
+    """(hi, lo) of the oracle's double-double sum of (hi, lo) pairs, in order."""
+    p = np.ascontiguousarray(pairs, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(2)
+    load_oracle().oracle_dd_sum_dd(p.ctypes.data, p.shape[0], out.ctypes.data)
+    return float(out[0]), float(out[1])
+
+
+def dd_mul_d(hi: float, lo: float, b: float) -> tuple:
+    """(hi, lo) of the oracle's double-double (hi, lo) times the double b."""
+    out = np.zeros(2)
+    load_oracle().oracle_dd_mul_d(hi, lo, b, out.ctypes.data)
+    return float(out[0]), float(out[1])
 
 
 def max_threads() -> int:

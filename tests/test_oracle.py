@@ -224,6 +224,144 @@ def test_compensated_oracle_close(oracle):
         np.testing.assert_allclose(s[k], d[k], rtol=1e-11)
 
 
+# ---- the compensated (double-double) oracle ---------------------------------
+#
+# oracle_sigma_dd sums the reference's rounded terms in double-double, so it
+# is the reference's arithmetic without the summation error of its
+# left-to-right sums (gppComplex.cpp:125-126, :146, :363, :379).  Its helpers
+# are checked against exact rational arithmetic, and the whole loop against
+# the exact-rational brute force on inputs where the sequential sum is far
+# off, so a TwoSum that loses its low part fails.
+
+def _fr_sum(xs):
+    return sum((exact_gpp.Fr(float(x)) for x in xs), exact_gpp.Fr(0))
+
+
+@pytest.mark.parametrize("xs,exact", [
+    ([2.0 ** 60, 1.0, -2.0 ** 60], 1.0),                 # the 1 is below the ulp of 2^60
+    ([1.0] + [2.0 ** -60] * 1000, 1.0 + 1000 * 2.0 ** -60),
+    ([2.0 ** 80, 3.0, -(2.0 ** 80), -1.5, 2.0 ** -40], 1.5 + 2.0 ** -40),
+])
+def test_dd_sum_exact_known(oracle, xs, exact):
+    hi, lo = oracle.dd_sum(xs)
+    assert exact_gpp.Fr(hi) + exact_gpp.Fr(lo) == _fr_sum(xs)
+    assert hi + lo == exact
+    # the plain left-to-right double sum loses the small part
+    plain = 0.0
+    for x in xs:
+        plain += x
+    assert plain != hi + lo
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_dd_sum_random_vs_exact(oracle, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(4000) * 2.0 ** rng.integers(-30, 30, 4000)
+    x = np.concatenate([x, -x[::-1] * (1 + 2.0 ** -30)])  # heavy cancellation
+    hi, lo = oracle.dd_sum(x)
+    ex = _fr_sum(x)
+    scale = float(np.abs(x).sum())
+    err = abs(float(exact_gpp.Fr(hi) + exact_gpp.Fr(lo) - ex))
+    plain = float(np.cumsum(x)[-1])
+    assert err <= 2.0 ** -100 * len(x) * scale
+    assert abs(plain - float(ex)) > 1e6 * max(err, 2.0 ** -200 * scale)
+
+
+def test_dd_sum_dd_and_mul_d_vs_exact(oracle):
+    rng = np.random.default_rng(7)
+    F = exact_gpp.Fr
+    pairs = []
+    for _ in range(500):
+        h = float(rng.standard_normal() * 2.0 ** int(rng.integers(-20, 40)))
+        l = float(h * 2.0 ** -53 * rng.uniform(-0.5, 0.5))  # |lo| <= ulp(hi)/2
+        pairs.append((h, l))
+    pairs += [(-h, -l * (1 - 2.0 ** -20)) for h, l in pairs[::-1]]
+    hi, lo = oracle.dd_sum_dd(pairs)
+    ex = sum((F(h) + F(l) for h, l in pairs), F(0))
+    scale = sum(abs(h) for h, _ in pairs)
+    assert abs(float(F(hi) + F(lo) - ex)) <= 2.0 ** -100 * len(pairs) * scale
+    # dropping every pair's lo part is wrong by far more than that
+    assert abs(float(sum((F(h) for h, _ in pairs), F(0)) - ex)) > 1e3 * 2.0 ** -100 * len(pairs) * scale
+    for h, l in pairs[:200]:
+        for b in (0.5 * 3.7, 1.5, float(rng.uniform(0.05, 5.0))):
+            rh, rl = oracle.dd_mul_d(h, l, b)
+            want = (F(h) + F(l)) * F(b)
+            assert abs(float(F(rh) + F(rl) - want)) <= 2.0 ** -104 * abs(h * b)
+    # the low part is carried: (2^60, 1) * 1.5 = 1.5 * 2^60 + 1.5 exactly
+    rh, rl = oracle.dd_mul_d(2.0 ** 60, 1.0, 1.5)
+    assert F(rh) + F(rl) == F(2) ** 60 * F(3, 2) + F(3, 2)
+
+
+def _dyadic_cancelling_inputs():
+    """Inputs whose every term the reference computes exactly (dyadic values,
+    wtilde = 1 and wx = 2 or 3 so |wdiff|^2 = 1 or 4 and delw = 1 or 1/2;
+    at wx = 2 branch 1 gives ssx = I_eps/4 exactly), with terms of 2^60 that
+    cancel inside a column's G sum (band 0) or across bands (1 and 2), so
+    the reference's left-to-right sums lose the small terms entirely."""
+    nb, ncouls, ngpown = 3, 8, 2
+    big = 2.0 ** 60
+    an = np.zeros((nb, ncouls), np.complex128)
+    an[0] = [big * (1 + 1j), 1 + 2j, -big * (1 + 1j), 3 - 1j, 2.0 ** -30, 0, 0.25j, -0.5]
+    # bands 1 and 2 leave +-2^60 in every column's G sum; they cancel only
+    # across bands, after the scaling by 0.5*vcoul (double-double times double)
+    an[1] = [big, 0.75 - 1j, big * 1j, 5.0, -big * 1j, 2.0 ** -20, 1j, 0]
+    an[2] = [-big, 1.5, 0.5 - 0.5j, 7j, -2.0, 2.0 ** -10, 1, 0]
+    return {
+        "inv_igp_index": np.array([0, 1], np.int32),
+        "wx_array": np.array([2.0, 3.0]),
+        "vcoul": np.array([3.0, 2.0] + [1.0] * (ncouls - 2)),   # 0.5*vcoul = 1.5 and 1
+        "aqsmtemp": np.ones((nb, ncouls), np.complex128),
+        "aqsntemp": an,
+        "I_eps_array": np.ones((ngpown, ncouls), np.complex128) * (1 + 0j),
+        "wtilde_array": np.ones((ngpown, ncouls), np.complex128),
+    }
+
+
+def test_compensated_oracle_exact_on_dyadic_cancellation(oracle):
+    a = _dyadic_cancelling_inputs()
+    ex = exact_gpp.sigma_exact(a, nv=1)
+    d = oracle.sigma_dd(a, nvband=1)
+    s = oracle.sigma(a, nvband=1)
+    worst_seq = 0.0
+    checked = 0
+    for k, iws in (("achtemp", (0, 1)), ("asxtemp", (0,))):  # asxtemp at wx=3 has inexact 1/81 terms
+        for iw in iws:
+            e = exact_gpp.to_complex(ex[k][iw])  # correctly rounded exact value
+            assert d[k][iw] == e, (k, iw, d[k][iw], e)  # the dd sum is exact, then rounded once
+            worst_seq = max(worst_seq, abs(s[k][iw] - e) / abs(e))
+            checked += 1
+    assert checked == 3
+    assert worst_seq > 1e-3  # the sequential sums lose the small terms
+    # asxtemp at wx = 3: terms carry the reference's rounding of 1/81, so dd
+    # is within the terms' own rounding of the exact value
+    e = exact_gpp.to_complex(ex["asxtemp"][1])
+    assert abs(d["asxtemp"][1] - e) <= 2e-16 * ex["asxtemp_mag"][1]
+
+
+@pytest.mark.parametrize("case", ["seeded", "near_pole", "cancel"])
+def test_compensated_oracle_vs_exact_rational(oracle, case):
+    if case == "seeded":
+        a, nv = gpp_inputs.seeded_inputs(4, 7, 3, nfreq=3, seed=11), 2
+    elif case == "near_pole":
+        a, nv = gpp_inputs.near_pole_inputs(3, 9, 4, seed=6), 2
+    else:
+        # each band's row followed by its negation times (1 + 2^-30): the G
+        # sums cancel to about 2^-30 of their terms
+        a, nv = gpp_inputs.seeded_inputs(3, 12, 2, nfreq=2, seed=3), 1
+        for key in ("aqsntemp",):
+            h = a[key][:, :6]
+            a[key][:, 6:] = -h * (1 + 2.0 ** -30)
+        a["I_eps_array"][:, 6:] = a["I_eps_array"][:, :6]
+        a["wtilde_array"][:, 6:] = a["wtilde_array"][:, :6]
+    ex = exact_gpp.sigma_exact(a, nv)
+    d = oracle.sigma_dd(a, nvband=nv)
+    for k in ("achtemp", "asxtemp"):
+        for iw in range(len(a["wx_array"])):
+            e = exact_gpp.to_complex(ex[k][iw])
+            scale = ex[k + "_mag"][iw] + 1e-300
+            assert abs(d[k][iw] - e) <= 2e-16 * scale, (case, k, iw, d[k][iw], e)
+
+
 # ---- the compiled reference --------------------------------------------------
 
 def _ref_or_skip(oracle):
