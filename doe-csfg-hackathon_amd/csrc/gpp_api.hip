@@ -173,7 +173,9 @@ void StageTimer::mark(hipStream_t s, const char *nm)
 // ---------------------------------------------------------------------------
 struct gpp_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the context enqueues: own_stream or the caller's (gpp_ctx_set_stream)
+    hipStream_t own_stream = nullptr;  // created with the context
+    int col_j0 = 0, col_j1 = 0;        // gpp_ctx_set_columns: evaluate G' columns [col_j0, col_j1) (0, 0: all)
     gpp_problem prob{};
     bool have_inputs = false;
     gpp::DevProblem dp;
@@ -270,6 +272,18 @@ gpp_status check_index_device(gpp_ctx *ctx, const int32_t *d_idx, int ngpown)
     return GPP_OK;
 }
 
+// Whether a device frequency row holds a negative entry (the factored reduce
+// picks its kernel by it): one small synchronous copy.
+gpp_status wx_negative_device(gpp_ctx *ctx, const double *d_wx, int nfreq, int *neg)
+{
+    std::vector<double> wx(nfreq);
+    HIP_OR_FAIL(hipMemcpyAsync(wx.data(), d_wx, sizeof(double) * nfreq, hipMemcpyDeviceToHost, ctx->stream),
+                "D2H wx_array");
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    *neg = gpp::wx_has_negative(wx.data(), nfreq);
+    return GPP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -296,7 +310,8 @@ gpp_status gpp_ctx_create(gpp_ctx **out, int device)
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    ctx->stream = ctx->own_stream;
     if (e == hipSuccess) e = ctx->timer.init();
     if (e != hipSuccess) {
         gpp_ctx_destroy(ctx);
@@ -316,7 +331,7 @@ gpp_status gpp_ctx_destroy(gpp_ctx *ctx)
     ctx->results.release(); ctx->flag.release(); ctx->wxb.release();
     ctx->work.release();
     ctx->timer.destroy();
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return GPP_OK;
 }
@@ -347,10 +362,12 @@ gpp_status gpp_upload(gpp_ctx *ctx, const gpp_problem *prob, const int32_t *inv_
     HIP_OR_FAIL(hipMemcpyAsync(ctx->ieps.ptr, I_eps_array, sizeof(cplx) * ng * nc, hipMemcpyHostToDevice, s), "H2D I_eps_array");
     HIP_OR_FAIL(hipMemcpyAsync(ctx->wtilde.ptr, wtilde_array, sizeof(cplx) * ng * nc, hipMemcpyHostToDevice, s), "H2D wtilde_array");
     HIP_OR_FAIL(hipStreamSynchronize(s), "upload synchronize");
+    ctx->dp.wx_neg = gpp::wx_has_negative(wx_array, (int)nf);
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
     ctx->have_last_form = false;
+    ctx->col_j0 = ctx->col_j1 = 0;
     return GPP_OK;
 }
 
@@ -450,10 +467,12 @@ gpp_status gpp_upload_file(gpp_ctx *ctx, const char *path, gpp_outputs *expected
     HIP_OR_FAIL(hipMemcpyAsync(ctx->wx.ptr, wx, sizeof(double) * p.nfreq, hipMemcpyHostToDevice, ctx->stream),
                 "H2D wx_array");
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "file upload synchronize");
+    ctx->dp.wx_neg = gpp::wx_has_negative(wx, p.nfreq);
     ctx->prob = p;
     ctx->have_inputs = true;
     ctx->per_band = false;
     ctx->have_last_form = false;
+    ctx->col_j0 = ctx->col_j1 = 0;
     if ((h.flags & 1u) && (st = gpp_set_band_frequencies(ctx, wx))) return st;
     if (expected && (h.flags & 2u)) {
         if (expected->achtemp) memcpy(expected->achtemp, sink.host[kAchtemp].data(), L.bytes[kAchtemp]);
@@ -480,8 +499,11 @@ gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t 
     if ((st = set_device(ctx))) return st;
     ctx->have_inputs = false;
     if ((st = check_index_device(ctx, d_inv_igp_index, prob->ngpown))) return st;
+    int neg = 0;
+    if ((st = wx_negative_device(ctx, d_wx_array, prob->nfreq, &neg))) return st;
     gpp::DevProblem &dp = ctx->dp;
     fill_dp_sizes(dp, prob);
+    dp.wx_neg = neg;
     dp.idx = d_inv_igp_index;
     dp.wx = d_wx_array;
     dp.vcoul = d_vcoul;
@@ -493,6 +515,7 @@ gpp_status gpp_bind_device(gpp_ctx *ctx, const gpp_problem *prob, const int32_t 
     ctx->have_inputs = true;
     ctx->per_band = false;
     ctx->have_last_form = false;
+    ctx->col_j0 = ctx->col_j1 = 0;
     return GPP_OK;
 }
 
@@ -601,10 +624,12 @@ gpp_status gpp_generate_seeded(gpp_ctx *ctx, const gpp_problem *prob, uint64_t s
     HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->ieps.ptr), ng * nc, seed, gpp::kArrIeps, -0.5, 0.5, -0.5, 0.5, s), "fill I_eps");
     HIP_OR_FAIL(gpp::launch_fill_seeded(static_cast<cplx *>(ctx->wtilde.ptr), ng * nc, seed, gpp::kArrWtilde, 0.5, 5.0, -0.05, 0.05, s), "fill wtilde");
     HIP_OR_FAIL(hipStreamSynchronize(s), "generate synchronize");
+    ctx->dp.wx_neg = gpp::wx_has_negative(wx.data(), prob->nfreq);
     ctx->prob = *prob;
     ctx->have_inputs = true;
     ctx->per_band = false;
     ctx->have_last_form = false;
+    ctx->col_j0 = ctx->col_j1 = 0;
     return GPP_OK;
 }
 
@@ -646,6 +671,42 @@ gpp_status gpp_ctx_last_form(const gpp_ctx *ctx, gpp_form *form)
     return GPP_OK;
 }
 
+gpp_status gpp_ctx_set_stream(gpp_ctx *ctx, struct ihipStream_t *stream)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    gpp_status st = set_device(ctx);
+    if (st) return st;
+    hipStream_t next = stream ? stream : ctx->own_stream;
+    if (next == ctx->stream) return GPP_OK;
+    if (stream) {  // the stream must belong to the context's device
+        int dev = -1;
+        HIP_OR_FAIL(hipStreamGetDevice(stream, &dev), "hipStreamGetDevice");
+        if (dev != ctx->device)
+            return fail(GPP_ERR_INVALID_ARGUMENT, "stream is on device %d, the context on device %d", dev,
+                        ctx->device);
+    }
+    // the context's earlier work (results, timing events) finishes first
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    ctx->stream = next;
+    return GPP_OK;
+}
+
+gpp_status gpp_ctx_set_columns(gpp_ctx *ctx, int j0, int j1)
+{
+    if (!ctx) return fail(GPP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->have_inputs) return fail(GPP_ERR_NO_INPUTS, "no inputs: call gpp_upload or gpp_bind_device first");
+    if (j0 == 0 && j1 == 0) {
+        ctx->col_j0 = ctx->col_j1 = 0;
+        return GPP_OK;
+    }
+    if (j0 < 0 || j1 <= j0 || j1 > ctx->dp.ngpown)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "columns [%d, %d) not within [0, %d)", j0, j1, ctx->dp.ngpown);
+    ctx->col_j0 = j0;
+    ctx->col_j1 = j1;
+    ctx->have_last_form = false;
+    return GPP_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -664,6 +725,13 @@ gpp_status sigma_enqueue(gpp_ctx *ctx, gpp_form form, bool acht)
     // the plain factored form on that row) and fall back to the direct form
     // beyond max_buckets groups.
     gpp::DevProblem dp = ctx->dp;
+    if (ctx->col_j1 > ctx->col_j0) {  // one slab of G' columns (gpp_ctx_set_columns)
+        const size_t off = (size_t)ctx->col_j0 * dp.ncouls;
+        dp.ngpown = ctx->col_j1 - ctx->col_j0;
+        dp.idx += ctx->col_j0;
+        dp.ieps += off;
+        dp.wtilde += off;
+    }
     bool bucketed = false;
     if (ctx->per_band) {
         const int nbk = (int)ctx->bplan.row.size();
@@ -1089,11 +1157,14 @@ gpp_status gpp_batch_create(gpp_ctx *ctx, int n, const gpp_problem *probs, const
             gpp_batch_destroy(b);
             return fail(GPP_ERR_INVALID_ARGUMENT, "instance %d: NULL or misaligned device array", i);
         }
-        if ((st = check_index_device(ctx, in.inv_igp_index, probs[i].ngpown))) {
+        int neg = 0;
+        if ((st = check_index_device(ctx, in.inv_igp_index, probs[i].ngpown)) ||
+            (st = wx_negative_device(ctx, in.wx_array, probs[i].nfreq, &neg))) {
             gpp_batch_destroy(b);
             return st;
         }
         gpp::DevProblem &dp = b->dps[i];
+        dp.wx_neg = neg;
         dp.idx = in.inv_igp_index;
         dp.wx = in.wx_array;
         dp.vcoul = in.vcoul;
@@ -1151,6 +1222,7 @@ gpp_status gpp_batch_create_seeded(gpp_ctx *ctx, int n, const gpp_problem *probs
             return st ? st : fail_hip(e, "batch input generation");
         }
         gpp::DevProblem &dp = b->dps[i];
+        dp.wx_neg = gpp::wx_has_negative(wx.data(), p.nfreq);
         dp.idx = static_cast<const int32_t *>(o[0].ptr);
         dp.wx = static_cast<const double *>(o[1].ptr);
         dp.vcoul = static_cast<const double *>(o[2].ptr);

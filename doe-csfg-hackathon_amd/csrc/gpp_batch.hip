@@ -143,9 +143,10 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
     oz::gram_tiles32<S, D, NBUF, LONG>(OzItemSeq{insts, items, nitems}, lds);
 }
 
-// Reduce items of instances whose occupied band count is NV (0: any count).
-// Occupancy as the single-problem reduce: two waves per SIMD at FC = 8.
-template <int FC, int NV>
+// Reduce items of instances whose occupied band count is NV (0: any count);
+// NEG: instances with a negative frequency (generic count).  Occupancy as the
+// single-problem reduce: two waves per SIMD at FC = 8.
+template <int FC, int NV, bool NEG = false>
 __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
                                                               const RedItem *__restrict__ items, int nitems,
                                                               int chunk)
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(c
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
         ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
         ra.fpb = b.nfreq;
-        pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC)>(ra, it.j, it.split, 0, b.nfreq, scratch);
+        pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC), NEG>(ra, it.j, it.split, 0, b.nfreq, scratch);
     }
 }
 
@@ -183,9 +184,11 @@ constexpr int gram_ti(int tile) { return tile == 0 || tile == 1 ? 128 : 64; }
 
 // ---------------------------------------------------------------------------
 // Reduce item classes: class c in 1..8 holds the instances with c occupied
-// bands (c >= 2 run kernels specialised on that count), class 0 the rest.
-constexpr int kRedClasses = 9;
-inline int red_class(int nv) { return nv >= 2 && nv <= 8 ? nv : 0; }
+// bands (c >= 2 run kernels specialised on that count), class 0 the rest,
+// class 9 the instances with a negative frequency (any count).
+constexpr int kRedClasses = 10;
+constexpr int kRedClassNeg = 9;
+inline int red_class(const DevProblem &p) { return p.wx_neg ? kRedClassNeg : p.nv >= 2 && p.nv <= 8 ? p.nv : 0; }
 
 BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes)
 {
@@ -250,7 +253,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         for (int j = 0; j < p.ngpown; ++j)
             for (int sp = 0; sp < nsplit; ++sp) {
                 const long len = std::min(k.red_chunk, p.ncouls - sp * k.red_chunk);
-                r[red_class(p.nv)].push_back({len * slots * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp}});
+                r[red_class(p)].push_back({len * slots * (p.nv > 0 ? 3 : 1), RedItem{i, j, sp}});
             }
         for (int t = 0; t < p.nb * kAchSplit; ++t) a.push_back(int2{i, t});
     }
@@ -397,6 +400,9 @@ void launch_batch_reduce(int cls, dim3 grid, hipStream_t s, const BInst *insts, 
     case 6: hipLaunchKernelGGL((k_batch_reduce<FC, 6>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
     case 7: hipLaunchKernelGGL((k_batch_reduce<FC, 7>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
     case 8: hipLaunchKernelGGL((k_batch_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case kRedClassNeg:
+        hipLaunchKernelGGL((k_batch_reduce<FC, 0, true>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk);
+        break;
     default: hipLaunchKernelGGL((k_batch_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
     }
 }

@@ -137,12 +137,12 @@ __device__ __forceinline__ cplx block_sum(cplx v, cplx *scratch)
 constexpr double kTo1 = 1e-6;    // to1, gppComplex.cpp:20, :241
 constexpr double kSexcut = 4.0;  // sexcut, gppComplex.cpp:21, :243
 
-// Occupied-band pole terms, ssxt_scht_solver (gppComplex.cpp:16-63), in the
-// reference's operation order with IEEE division and no contraction, so the
-// branch decisions are taken on the same values as the reference.
-// wx2 = wxt^2 and rwx4 = 1/wx2^2 (:38-40) are passed in.
-__device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rwx4, cplx wt, cplx ie,
-                                               cplx &sch, cplx &ssx)
+// Occupied-band pole terms before the ssx cutoff, ssxt_scht_solver
+// (gppComplex.cpp:16-56), in the reference's operation order with IEEE
+// division and no contraction, so the branch decisions and values are the
+// reference's.  wx2 = wxt^2 and rwx4 = 1/wx2^2 (:38-40) are passed in.
+__device__ __forceinline__ void occ_pole_terms_uncut(double wxt, double wx2, double rwx4, cplx wt, cplx ie,
+                                                     cplx &sch, cplx &ssx)
 {
 #pragma clang fp contract(off)
     const double limitone = 1.0 / (kTo1 * 4.0);  // :22
@@ -164,7 +164,31 @@ __device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rw
         const double rd = 1.00 / cmul(cden, cconj(cden)).re;            // :48-49
         ssx = cmul(cmul(cplx{-om2.re, -om2.im}, cconj(cden)), cscale(delw, rd));  // :50
     }
-    // ssx cutoff, :58-59 (only reachable for wx < 0)
+}
+
+// |ssx| of the reference's branch 2 (gppComplex.cpp:44-50) for one element,
+// in its operation order: the rare near-cutoff path of the factored reduce,
+// kept out of line so that it does not add to the reduce loop's registers.
+__device__ __attribute__((noinline)) double occ_branch2_abs_ssx(double wxt, cplx wt, cplx ie)
+{
+#pragma clang fp contract(off)
+    const cplx wt2 = csquare(wt);                                          // :117
+    const cplx om2 = cmul(wt2, ie);                                        // :118
+    const cplx wdiff = cdminus(wxt, wt);                                   // :27
+    const double rden = 1 / cmul(wdiff, cconj(wdiff)).re;                  // :30
+    const cplx delw = cscale(cmul(wt, cconj(wdiff)), rden);                // :31
+    const cplx cden = cscale(cmul(wt2, cdplus(0.50, delw)), 4.00);         // :46
+    const double rd = 1.00 / cmul(cden, cconj(cden)).re;                   // :48-49
+    return cabs_(cmul(cmul(cplx{-om2.re, -om2.im}, cconj(cden)), cscale(delw, rd)));  // :50, :59
+}
+
+// ssxt_scht_solver (gppComplex.cpp:16-63) with the ssx cutoff of :58-59
+// (only reachable for wx < 0).
+__device__ __forceinline__ void occ_pole_terms(double wxt, double wx2, double rwx4, cplx wt, cplx ie,
+                                               cplx &sch, cplx &ssx)
+{
+#pragma clang fp contract(off)
+    occ_pole_terms_uncut(wxt, wx2, rwx4, wt, ie, sch, ssx);
     if (wxt < 0.00) {
         const double ssxcutoff = cabs_(ie) * kSexcut;
         if (cabs_(ssx) > ssxcutoff) ssx = czero();

@@ -106,13 +106,13 @@ __global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 
 
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
 // compile time (0: any).
-template <int FC, int NV, bool P2>
+template <int FC, int NV, bool P2, bool NEG = false>
 __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
     const int iw_end = iw_begin + a.fpb < a.nfreq ? iw_begin + a.fpb : a.nfreq;
-    pole_reduce_item<FC, NV, P2, (FC > 4 ? 4 : FC)>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
+    pole_reduce_item<FC, NV, P2, (FC > 4 ? 4 : FC), NEG>(a, blockIdx.x, blockIdx.y, iw_begin, iw_end, scratch);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_fact_final(const cplx *col_ach, const cplx *col_asx,
@@ -210,9 +210,14 @@ void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t 
 }
 
 // P2: also store the pole factor (and occupied branch) of frequency acht_iw.
+// Problems with a negative frequency run the generic NEG kernel.
 template <int FC, bool P2>
 void launch_reduce_p2(const ReduceArgs &a, dim3 grid, hipStream_t s)
 {
+    if (a.neg) {
+        hipLaunchKernelGGL((k_pole_reduce<FC, 0, P2, true>), grid, dim3(kRedThreads), 0, s, a);
+        return;
+    }
     switch (a.nv) {  // the occupied band counts of the common shapes
     case 2: hipLaunchKernelGGL((k_pole_reduce<FC, 2, P2>), grid, dim3(kRedThreads), 0, s, a); break;
     case 4: hipLaunchKernelGGL((k_pole_reduce<FC, 4, P2>), grid, dim3(kRedThreads), 0, s, a); break;
@@ -302,6 +307,7 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     ra.p2 = kind.acht ? reinterpret_cast<cplx *>(w + L.p2) : nullptr;
     ra.br1 = kind.acht && p.nv > 0 ? reinterpret_cast<uint8_t *>(w + L.br1) : nullptr;
     ra.acht_iw = p.nfreq > 2 ? 2 : p.nfreq - 1;  // sch_array[2], gppComplex.cpp:381
+    ra.neg = p.wx_neg;
     GPP_TRY(launch_pole_reduce(ra, kind.red_fc, s));
     if (timer) timer->mark(s, "factored_pole_reduce");
     hipLaunchKernelGGL(k_fact_final, dim3(p.nfreq + 1), dim3(kRedThreads), 0, s, col_ach, col_asx, part_achs,
@@ -386,6 +392,7 @@ DevProblem bucket_problem(const DevProblem &p, const BucketPlan &bp, int b, cons
     q.aqsn = aqsn_p + (size_t)bp.start[b] * p.ncouls;
     q.wx = p.wx + (size_t)bp.row[b] * p.nfreq;
     q.wx_stride = 0;
+    q.wx_neg = bp.neg.empty() ? 1 : bp.neg[b];
     return q;
 }
 
@@ -425,6 +432,7 @@ BucketPlan plan_buckets(const double *wx_rows, int nb, int nv, int nfreq)
         if (bucket[n1] < 0) {
             bucket[n1] = (int)bp.row.size();
             bp.row.push_back(n1);
+            bp.neg.push_back(wx_has_negative(r, nfreq));
         }
     }
     const int nbk = (int)bp.row.size();

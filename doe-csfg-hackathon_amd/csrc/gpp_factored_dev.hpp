@@ -162,6 +162,7 @@ struct ReduceArgs {
     cplx *p2;
     uint8_t *br1;
     int acht_iw;
+    int neg;  // some frequency is negative: launch the NEG variant (host-side dispatch only)
 };
 
 // Stage 3: pole terms of column j over G-vectors [split*chunk, +chunk), for
@@ -175,8 +176,11 @@ struct ReduceArgs {
 // partials (0.5*sum)*vcoul and sum*(occ*vcoul).  scratch: 4 entries.
 // NV > 0: the occupied band count is NV (== a.nv), known at compile time, so
 // the occupied loads are unrolled and issued together and conj(bg) of the
-// column is loaded once; NV == 0: any a.nv.
-template <int FC, int NV = 0, bool P2 = false, int FCT = FC>
+// column is loaded once; NV == 0: any a.nv.  NEG: some frequency may be
+// negative, where the ssx cutoff of :58-59 is live; without it every
+// frequency must be >= 0 (the launcher checks on the host) and the kernel
+// carries no cutoff code.
+template <int FC, int NV = 0, bool P2 = false, int FCT = FC, bool NEG = false>
 __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw_begin, int iw_end,
                                                  cplx *scratch)
 {
@@ -217,8 +221,10 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         for (int f = 0; f < F; ++f) sach[f] = sasx[f] = czero();
 
         bool any_neg = false;  // the ssx cutoff of :58-59 applies only at negative frequencies
+        if constexpr (NEG) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) any_neg = any_neg || wxv[f] < 0.00;
+            for (int f = 0; f < F; ++f) any_neg = any_neg || wxv[f] < 0.00;
+        }
 
         const size_t row = (size_t)j * ncouls;
         // The next element's wtilde, I_eps and Mu load while this one is
@@ -280,7 +286,7 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                     mum = cadd(mu, mo);
                 }
                 wt2 = __builtin_fma(wt.re, wt.re, wti2);
-                if (any_neg) {
+                if (NEG && any_neg) {
                     // A real (uniform) branch: the empty asm keeps the compiler
                     // from evaluating the square root for every element.
                     asm volatile("" ::: "memory");
@@ -326,7 +332,8 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 //                                     = -(q / 2) / (wx + wt),
                 // a pole at -wt, far from cancellation for wx > 0, with
                 // wx + wt = (s, wt.im).
-                if (!FOLD || wxv[f] < 0.00) {
+                const bool neg_f = NEG && wxv[f] < 0.00;
+                if (!FOLD || neg_f) {
                     const double q_im_wti = FOLD ? q.im * wt.im : q_im_wti_e;
                     const double q_re_wti = FOLD ? q.re * wt.im : q_re_wti_e;
                     const cplx p = {__builtin_fma(q.re, wdr, -q_im_wti) * rden,
@@ -340,8 +347,28 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                             const double rr = 0.5 * rcp_1nr(__builtin_fma(s, s, wti2));
                             ssx = cplx{-__builtin_fma(q.re, s, q_im_wti) * rr, -__builtin_fma(q.im, s, -q_re_wti) * rr};
                         }
-                        // the ssx cutoff of :58-59 (only at negative frequencies)
-                        if (wxv[f] < 0.00 && cabs_(ssx) > ie_cut) ssx = czero();
+                        // The ssx cutoff of :58-59 (only at negative
+                        // frequencies), decided on the reference's |ssx|.
+                        // Branch 1: the reference's ssx is ((Omega2 wx^2)
+                        // (1/wx^4)) (:38-41) with Omega2 already in its
+                        // operation order.  Branch 2: this ssx is within about
+                        // 2^-45 of the reference's, relative (rcp_1nr, the
+                        // reduced form), so outside a 2^-40 band around the
+                        // cutoff it decides as the reference does; inside it
+                        // the reference's ssx is recomputed and decides.
+                        if (neg_f) {
+                            bool cut;
+                            if (branch1) {
+                                const double wx2 = wxv[f] * wxv[f];
+                                cut = cabs_(cscale2(om2, wx2, 1.00 / (wx2 * wx2))) > ie_cut;
+                            } else {
+                                const double ax = cabs_(ssx);
+                                cut = ax > ie_cut;
+                                if (__builtin_fabs(ax - ie_cut) <= kBand * ie_cut)
+                                    cut = occ_branch2_abs_ssx(wxv[f], wt, ie) > ie_cut;
+                            }
+                            if (cut) ssx = czero();
+                        }
                         sasx[f] = cfma(ssx, mo, sasx[f]);
                     }
                     if constexpr (FOLD) {

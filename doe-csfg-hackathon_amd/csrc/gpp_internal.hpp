@@ -19,6 +19,8 @@ struct DevProblem {
     const int32_t *idx = nullptr;  // [ngpown]
     const double *wx = nullptr;    // [nfreq], or [nb][nfreq] when wx_stride == nfreq
     int wx_stride = 0;             // 0: one wx row for every band; nfreq: a row per band
+    int wx_neg = 0;                // the shared row has a negative frequency (ssx cutoff of gppComplex.cpp:58-59
+                                   // live), known on the host; the factored reduce selects its kernel by it
     const double *vcoul = nullptr; // [ncouls]
     const cplx *aqsm = nullptr;    // [nb][ncouls]
     const cplx *aqsn = nullptr;    // [nb][ncouls]
@@ -70,6 +72,15 @@ struct StageTimer {
     const Slot &last() const { return slot[cur]; }
     const Slot &back(int k) const { return slot[(cur - k + kRing) % kRing]; }  // k evaluations before the last
 };
+
+// 1 if some of the n frequencies is negative (NaN counts as negative, so an
+// unordered value takes the kernels that check every frequency).
+inline int wx_has_negative(const double *wx, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (!(wx[i] >= 0.0)) return 1;
+    return 0;
+}
 
 // Bytes of device workspace the direct form needs for this problem.
 size_t direct_workspace_bytes(const DevProblem &p);
@@ -129,6 +140,7 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
 struct BucketPlan {
     std::vector<int32_t> perm;
     std::vector<int> start, size, nocc, row;
+    std::vector<int> neg;      // per bucket: its row has a negative frequency
     std::vector<int32_t> tab;  // per bucket: result offset, start, size
     size_t nres = 0;           // complex results of all buckets (DevResults each)
 };
@@ -168,7 +180,7 @@ struct BatchPlan {
     std::vector<InstOff> inst_off;
     std::vector<GramItem> gram_items;
     std::vector<RedItem> red_items;  // grouped by occupied band class (plan_batch), each class by cost
-    int red_class_start[10] = {};    // items of class c: [red_class_start[c], red_class_start[c + 1])
+    int red_class_start[11] = {};    // items of class c: [red_class_start[c], red_class_start[c + 1])
     std::vector<int2> ach_items;
     // Ozaki kinds: column exponents (inst, side, 64-column block), digit
     // pieces (inst, side, 256-column block, piece) and 64 x 64 Gram tiles.

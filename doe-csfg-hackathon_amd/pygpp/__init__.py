@@ -66,6 +66,8 @@ EXPORTED_SYMBOLS = (
     "gpp_sigma",
     "gpp_sigma_async",
     "gpp_ctx_wait",
+    "gpp_ctx_set_stream",
+    "gpp_ctx_set_columns",
     "gpp_ctx_timing_summary",
     "gpp_set_band_frequencies",
     "gpp_file_write",
@@ -165,6 +167,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gpp_sigma.argtypes = [vp, i32, P(GppOutputs)]
     lib.gpp_sigma_async.argtypes = [vp, i32, i32]
     lib.gpp_ctx_wait.argtypes = [vp, P(GppOutputs)]
+    lib.gpp_ctx_set_stream.argtypes = [vp, vp]
+    lib.gpp_ctx_set_columns.argtypes = [vp, i32, i32]
     lib.gpp_ctx_timing_summary.argtypes = [vp, i32, P(i32), P(dbl), P(dbl), i32, P(i32)]
     lib.gpp_set_band_frequencies.argtypes = [vp, vp]
     lib.gpp_file_write.argtypes = [ctypes.c_char_p, P(GppProblem), vp, vp, i32] + [vp] * 5 + [P(GppOutputs)]
@@ -190,7 +194,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.d_till_nvband.argtypes = [i32, i32, i32] + [vp] * 11 + [dbl]
     lib.d_reduce_achstemp.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, i32, vp]
     lib.d_noFlagOCCSolver.argtypes = [i32, i32, i32, i32] + [vp] * 10
-    for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device",
+    for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device", "gpp_ctx_set_stream",
+                 "gpp_ctx_set_columns", "gpp_sigma_async", "gpp_ctx_wait",
                  "gpp_fill_deterministic", "gpp_fill_seeded", "gpp_generate_seeded", "gpp_sigma",
                  "gpp_ctx_last_timing", "gpp_ctx_set_option", "gpp_ctx_get_option", "gpp_ctx_autotune",
                  "gpp_batch_create", "gpp_batch_create_seeded", "gpp_batch_sigma", "gpp_batch_autotune",
@@ -447,6 +452,16 @@ class Context:
         _check(load().gpp_ctx_wait(self._h, ctypes.byref(o)))
         out["achstemp"] = out["achstemp"][0]
         return out
+
+    def set_stream(self, stream: Optional[int]) -> None:
+        """Enqueues the context's work on a caller-owned HIP stream (its
+        integer handle); None returns to the context's own stream."""
+        _check(load().gpp_ctx_set_stream(self._h, stream))
+
+    def set_columns(self, j0: int = 0, j1: int = 0) -> None:
+        """Evaluates only the G' columns [j0, j1) of the current inputs (one
+        rank's slab); (0, 0) restores every column."""
+        _check(load().gpp_ctx_set_columns(self._h, j0, j1))
 
     def timing_summary(self, last_n: int) -> Dict[str, float]:
         """Average GPU milliseconds of the last evaluations of the same kind:

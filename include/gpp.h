@@ -328,6 +328,34 @@ gpp_status gpp_sigma_async(gpp_ctx *ctx, gpp_form form, int with_acht_n1_loc);
  * outputs into out (may be NULL: only wait; NULL members are skipped). */
 gpp_status gpp_ctx_wait(gpp_ctx *ctx, const gpp_outputs *out);
 
+/*
+ * Enqueues the context's work on the caller's HIP stream (hipStream_t of the
+ * context's device) instead of its own: uploads, evaluations, timing events,
+ * result copies and batch replays.  Work the caller enqueued on that stream
+ * before gpp_sigma_async runs before the evaluation, and the caller's later
+ * work runs after it, so a host application that owns its streams orders the
+ * library's work with its own without synchronising (the reference's
+ * launchers run on the default stream and leave synchronisation to the
+ * caller, gppCudaVer.cpp:381-395).  The context first waits for the work
+ * already enqueued where it ran so far.  NULL returns to the context's own
+ * stream.  The caller keeps the stream alive until then or until
+ * gpp_ctx_destroy.
+ */
+struct ihipStream_t;
+gpp_status gpp_ctx_set_stream(gpp_ctx *ctx, struct ihipStream_t *stream);
+
+/*
+ * Restricts the following evaluations to the G' columns [j0, j1) of the
+ * current inputs (rows j0..j1-1 of I_eps_array / wtilde_array and entries of
+ * inv_igp_index): the slab one rank owns in the reference's MPI build
+ * (gppMPIOpenMP3.cpp:172, 255-271).  The outputs are that slab's partial
+ * sums (achtemp, asxtemp, acht_n1_loc and achstemp are sums over columns);
+ * adding the slabs' outputs in rank order gives the whole problem's, as the
+ * reference's MPI_Reduce does (gppMPIOpenMP3.cpp:361-370), up to the order
+ * of the additions.  j0 = j1 = 0 restores every column; new inputs reset it.
+ */
+gpp_status gpp_ctx_set_columns(gpp_ctx *ctx, int j0, int j1);
+
 /* Sets / reads a context option (gpp_option). */
 gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value);
 gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value);

@@ -140,3 +140,60 @@ def per_band_wx(nb, nfreq, seed=0, levels=None, e_lk=10.0, dw=1.0):
         e = rng.choice(rng.uniform(2.0, 9.0, levels), size=nb)
     wx = e_lk - e[:, None] + dw * (np.arange(nfreq)[None, :] - 1.0)
     return np.maximum(wx, 1e-6)
+
+
+def ref_abs_ssx(wx, wt, ie, branch):
+    """|ssx| before the cutoff and the cutoff 4|I_eps| of the occupied-band
+    solver for branch 1 (gppComplex.cpp:38-41) or 2 (:46-50), one IEEE
+    operation at a time in the reference's order (Complex.h:141-250)."""
+    wr, wi = np.real(wt), np.imag(wt)
+    er, ei = np.real(ie), np.imag(ie)
+    w2r, w2i = wr * wr - wi * wi, 2 * wr * wi                     # square, :117
+    o2r, o2i = w2r * er - w2i * ei, w2r * ei + w2i * er           # Omega2, :118
+    if branch == 1:
+        cdend = wx * wx                                           # pow(wx, 2), :38
+        rden = 1.00 / (cdend * cdend)                             # :39-40
+        sr, si = o2r * cdend * rden, o2i * cdend * rden           # mult(a, b, c), :41
+    else:
+        cr, ci = wx - wr, 0 - wi                                  # wdiff = cden, :27-29
+        rden = 1 / (cr * cr - ci * (-1 * ci))                     # :30
+        dr = (wr * cr - wi * (-1 * ci)) * rden                    # delw, :31
+        di = (wr * (-1 * ci) + wi * cr) * rden
+        pr, pi_ = 0.50 + dr, 0 + di                               # doublePlus(0.5, delw)
+        c2r, c2i = (w2r * pr - w2i * pi_) * 4.00, (w2r * pi_ + w2i * pr) * 4.00   # :46
+        rd = 1.00 / (c2r * c2r - c2i * (-1 * c2i))                # :48-49
+        xr = (-o2r) * c2r - (-o2i) * (-1 * c2i)                   # -Omega2 * conj(cden)
+        xi = (-o2r) * (-1 * c2i) + (-o2i) * c2r
+        yr, yi = dr * rd, di * rd                                 # delw * rden
+        sr, si = xr * yr - xi * yi, xr * yi + xi * yr             # :50
+    return np.sqrt(sr * sr + si * si), np.sqrt(er * er + ei * ei) * 4.0
+
+
+def ssx_cutoff_inputs(nb, ncouls, ngpown, seed=0, spread=12):
+    """Seeded inputs with wx = (-0.6, -0.2, 3.0) and wtilde placed so that the
+    occupied solver's |ssx| lies within a few ulps of the cutoff 4|I_eps| of
+    gppComplex.cpp:58-59 (live only at negative frequencies):
+      kind 0  branch 1 at wx = -0.6 with |wt|^2 / wx^2 ~ 4 (|wt| ~ 1.2)
+      kind 1  branch 2 at wx = -0.2 with |wt| / |wx + wt| ~ 8
+      kind 2  default seeded wtilde
+    Each placed modulus is scaled by (1 + k 2^-52), k uniform in
+    [-spread, spread], so the cutoff falls on both sides."""
+    a = seeded_inputs(nb, ncouls, ngpown, 3, seed)
+    rng = np.random.default_rng(seed + 3000)
+    shape = a["wtilde_array"].shape
+    kind = rng.integers(0, 3, shape)
+    jitter = 1.0 + rng.integers(-spread, spread + 1, shape) * 2.0 ** -52
+    th0 = rng.uniform(-2.5, 2.5, shape)                 # |-0.6 - wt|^2 = 1.8 + 1.44 cos(th) > 0.25
+    wt0 = 1.2 * np.exp(1j * th0) * jitter
+    wx1, t = -0.2, 8.0
+    th1 = rng.uniform(0.02, 0.11, shape) * rng.choice([-1.0, 1.0], shape)
+    aa, bb, cc = 1 - 1 / t ** 2, 2 * wx1 * np.cos(th1), wx1 * wx1
+    r1 = (-bb + np.sqrt(bb * bb - 4 * aa * cc)) / (2 * aa)   # |wx + r e^{ith}| = r / t
+    wt1 = r1 * np.exp(1j * th1) * jitter
+    wt = a["wtilde_array"]
+    wt = np.where(kind == 0, wt0, wt)
+    wt = np.where(kind == 1, wt1, wt)
+    a["wtilde_array"] = wt
+    a["wx_array"] = np.array([-0.6, -0.2, 3.0])
+    a["_kind"] = kind
+    return a

@@ -57,6 +57,24 @@ def test_factored_branch_thresholds(ctx, oracle, form):
     assert_close(ctx.sigma(form, acht_n1_loc=True), ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
 
 
+@pytest.mark.parametrize("form", FACTORED + ["direct", "direct-strict"])
+def test_ssx_cutoff_straddle(ctx, oracle, form):
+    """Two thirds of the elements put the reference's |ssx| within a few ulps
+    of the cutoff 4|I_eps| of gppComplex.cpp:58-59 at a negative frequency, on
+    both sides, in branch 1 (wx = -0.6) and branch 2 (wx = -0.2).  The
+    factored reduce decides branch 1 on the reference's own rounding of ssx
+    and branch 2 on its estimate outside a 2^-40 band (the reference's ssx
+    recomputed inside it): every decision, and so every output, matches the
+    oracle (one wrong decision moves asxtemp by far more than 1e-10)."""
+    arrs = gpp_inputs.ssx_cutoff_inputs(24, 2000, 60, seed=5)
+    arrs.pop("_kind")
+    ref = oracle.sigma(arrs, nvband=4, threads=8)
+    tol = 1e-12 if form.startswith("direct") else TOL_FACTORED
+    assert_close(run_gpu(ctx, arrs, 4, form), ref, tol, KEYS)
+    if not form.startswith("direct"):
+        assert_close(ctx.sigma(form, acht_n1_loc=True), ref, tol, KEYS + ("acht_n1_loc",))
+
+
 NEG_WX = {"nfreq3": [-2.0, -0.5, 3.0], "nfreq4": [-2.0, -0.5, 1.0, 3.0],
           "nfreq5": [-3.0, -0.7, 0.4, 2.0, 4.0]}
 

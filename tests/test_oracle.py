@@ -122,6 +122,39 @@ def test_threshold_inputs_straddle_thresholds():
         assert np.all((wdiffr > 0.25) == above_quarter)
 
 
+def test_ssx_cutoff_inputs_straddle_cutoff(oracle):
+    """The negative-frequency inputs the factored-form tests use put the
+    reference's |ssx| within 2^-40 of the cutoff 4|I_eps| (gppComplex.cpp:
+    58-59) for thousands of elements on each side, in branch 1 (wx = -0.6)
+    and branch 2 (wx = -0.2); the numpy restatement of the reference's ssx
+    agrees with the oracle's solver, and a branch-1 ssx rounded as
+    Omega2 * (wx^2 * (1/wx^4)) instead of (Omega2 * wx^2) * (1/wx^4) decides
+    hundreds of them the other way."""
+    a = gpp_inputs.ssx_cutoff_inputs(24, 2000, 60, seed=5)
+    kind, wt, ie = a["_kind"], a["wtilde_array"], a["I_eps_array"]
+    for k, wx, br in ((0, -0.6, 1), (1, -0.2, 2)):
+        s, cut = gpp_inputs.ref_abs_ssx(wx, wt[kind == k], ie[kind == k], br)
+        band = np.abs(s / cut - 1) < 2.0 ** -40
+        assert (s[band] > cut[band]).sum() > 5000 and (s[band] <= cut[band]).sum() > 5000
+        delwr, wdiffr = gpp_inputs.ref_delwr(wx, wt[kind == k])
+        if br == 1:
+            assert np.all((wdiffr > 0.25) & (delwr < 2.5e5))
+        else:
+            assert np.all(((wdiffr <= 0.25) | (delwr >= 2.5e5)) & (delwr > 1e-6))
+        for i in np.flatnonzero((kind == k).ravel())[:200]:
+            w, e = wt.ravel()[i], ie.ravel()[i]
+            got_br, _, ssx = oracle.pole_terms(wx, w, e)
+            s1, c1 = gpp_inputs.ref_abs_ssx(wx, np.array([w]), np.array([e]), br)
+            assert got_br == br and (ssx == 0) == (s1[0] > c1[0])
+        if br == 1:
+            wr, wi, er, ei = wt[kind == k].real, wt[kind == k].imag, ie[kind == k].real, ie[kind == k].imag
+            w2r, w2i = wr * wr - wi * wi, 2 * wr * wi
+            o2r, o2i = w2r * er - w2i * ei, w2r * ei + w2i * er
+            c = (wx * wx) * (1.00 / ((wx * wx) * (wx * wx)))
+            alt = np.sqrt((o2r * c) ** 2 + (o2i * c) ** 2)
+            assert ((alt > cut) != (s > cut)).sum() > 100
+
+
 def test_exact_rational_det(oracle):
     _check_exact(oracle, gpp_inputs.det_inputs(4, 2, 6, ngpown=3), 2)
 
