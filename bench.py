@@ -1,18 +1,23 @@
 #!/usr/bin/env python3
 """GPP sigma throughput on one MI355X.
 
-One step is one full evaluation of the GPP sigma band loop (gppComplex.cpp
-:335-383) -- achtemp/asxtemp/acht_n1_loc/achstemp -- over the benchmark
-problem resident in HBM: number_bands=512, nvband=2, ncouls=32768,
-ngpown=1638, nfreq=3, seeded complex128 inputs (generated on the device).
-The metric counts one update per (n1, igp, ig, iw) tuple.
+One step is one evaluation of the GPP sigma band loop (gppComplex.cpp
+:335-383) over the benchmark problem resident in HBM: number_bands=512,
+nvband=2, ncouls=32768, ngpown=1638, nfreq=3, seeded complex128 inputs
+(generated on the device).  The headline step computes achtemp, asxtemp and
+achstemp (the outputs the metric names); "factored-f64+acht" in `forms` is
+the same step with acht_n1_loc as well, every output of the reference's
+loop.  The metric counts one update per (n1, igp, ig, iw) tuple.
 
 Prints ONE JSON line.  `value` is the updates/s of the headline form; the
 other forms' numbers, their errors, per-stage GPU times (HIP events on the
-library's stream), the roofline of the dominant kernel and the reference's
-OpenMP CPU path timed on this host are attached.
+library's stream), the roofline of the dominant kernel, one G'-column slab's
+time and the reference's OpenMP CPU path timed on this host are attached.
+The GPU work runs first; the CPU leg then times the reference and, at the
+benchmark shape, runs the CPU oracle over the whole problem as the checker
+of every form's outputs (``rel_err_vs_cpu_oracle``).
 
-    python bench.py [--steps K] [--warmup W] [--form direct|...] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--form direct|...] [--no-cpu] [--no-cpu-parity]
 """
 import argparse
 import json
@@ -48,14 +53,30 @@ FLOP_UNOCC = 33           # per unoccupied update, counted on the reference's ex
 FLOP_OCC = 62             # per occupied update (branch-1 path)
 
 
+def host_cores():
+    """Cores this process may run on (its affinity mask)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# The CPU legs use at most 16 threads: a GPU box gives one GPU's job a share
+# of 16 host cores, whatever the machine's core count and affinity show.
+CPU_THREAD_CAP = 16
+
+
 def cpu_cores():
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
         return int(env)
-    try:
-        return min(16, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        return min(16, os.cpu_count() or 1)
+    return min(CPU_THREAD_CAP, host_cores())
+
+
+def cpu_core_note(cores):
+    return {"host_cores_available": host_cores(), "threads_used": cores,
+            "thread_cap": CPU_THREAD_CAP,
+            "cap_reason": "one GPU's share of the host is 16 cores (OMP_NUM_THREADS overrides)"}
 
 
 def cpu_baseline():
@@ -81,7 +102,7 @@ def cpu_baseline():
                     "sample": f"reference gppOpenMP3 (g++ -O3 -fopenmp, {cores} threads) on '{nb_s} {nv_s} {NCOULS} 20': "
                               f"{nb_s} of {NB} bands, ngpown={NCOULS // 20}, nfreq=3, its deterministic init; "
                               f"{updates:.3e} updates in {secs:.3f} s (its own loop timer)",
-                    "seconds": secs}
+                    "seconds": secs, "cores_detail": cpu_core_note(cores)}
     sys.path.insert(0, ORACLE)
     import gpp_oracle
     import pygpp
@@ -93,10 +114,34 @@ def cpu_baseline():
     return {"value": updates / secs, "unit": "updates/s", "cores": cores, "kind": "port",
             "sample": f"oracle restatement of gppComplex.cpp (race-free OpenMP, {cores} threads) on {nb_s} of {NB} "
                       f"bands at ncouls={NCOULS}, ngpown={NGPOWN}, nfreq=3, seeded; {updates:.3e} updates in {secs:.3f} s",
-            "seconds": secs}
+            "seconds": secs, "cores_detail": cpu_core_note(cores)}
 
 
-def run_c4(args, cpu):
+def cpu_parity(results):
+    """The checker half of the CPU leg: the CPU oracle (the reference's loop
+    restated in C, bitwise equal to the compiled reference on its fixtures)
+    over the whole benchmark problem on the host's cores, and every GPU
+    form's largest per-component relative error against it."""
+    sys.path.insert(0, ORACLE)
+    import gpp_oracle
+    import pygpp
+    cores = cpu_cores()
+    p = pygpp.Problem(NB, NV, NCOULS, NGPOWN, NFREQ)
+    arrs = pygpp.fill_seeded(p, SEED)  # the device generator's bits
+    t0 = time.perf_counter()
+    ref = gpp_oracle.sigma(arrs, nvband=NV, threads=cores)
+    secs = time.perf_counter() - t0
+    del arrs
+    errs = {}
+    for form, out in results.items():
+        keys = ("achtemp", "asxtemp", "achstemp") + (("acht_n1_loc",) if out.get("acht_n1_loc") is not None else ())
+        errs[form] = {k: rel_err(out[k], ref[k]) for k in keys}
+    return {"reference": f"CPU oracle (gpp_oracle.c, race-free OpenMP, {cores} threads) over all "
+                         f"{p.updates:.3e} updates of the benchmark shape, seed {SEED}",
+            "seconds": secs, "rel_err_per_component": errs}
+
+
+def run_c4(args):
     import numpy as np
     import pygpp
     from gpp_workloads import c4_instances
@@ -158,6 +203,7 @@ def run_c4(args, cpu):
         oz[f]["speedup_vs_direct"] = direct_ms / oz[f]["ms_per_step"]
     batch.close()
     ctx.close()
+    cpu = None if args.no_cpu else cpu_baseline()  # after the GPU work
     # executed work of the 3M complex product: three real products (6 flop)
     # per complex multiply-add; the 4-product (ZGEMM) count is 8
     gram_flop = sum(F64_FLOP_PER_CMAC * p.ngpown * p.ncouls * (p.number_bands - p.nvband) for p in probs)
@@ -248,7 +294,9 @@ def main():
                     help="untimed steps before them (default: 10 at c1, 3 at c3 and c4)")
     ap.add_argument("--form", default="factored-f64",
                     choices=["direct", "direct-strict", "factored-f64", "factored-ozaki", "factored-ozaki-i8"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and full-size parity)")
+    ap.add_argument("--no-cpu-parity", action="store_true",
+                    help="skip the CPU oracle over the whole benchmark problem (about 30 s on 16 cores)")
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     args = ap.parse_args()
     # A few milliseconds of GPU work after an idle host wait run at a lower
@@ -269,9 +317,8 @@ def main():
               "multi-GPU sharding of the G' columns is a design note, not run here.", file=sys.stderr)
         sys.exit(2)
 
-    cpu = None if args.no_cpu else cpu_baseline()
     if args.workload == "c4":
-        run_c4(args, cpu)
+        run_c4(args)
         return
 
     import pygpp
@@ -303,14 +350,25 @@ def main():
         results[form] = out
         forms[form] = {"ms_per_step": ms, "updates_per_s": updates / (ms * 1e-3), "gpu_stage_ms": stages}
     head = forms[args.form]
-    # Agreement with the strict direct form (reference operation order, FP64; the
-    # direct forms are themselves checked against the CPU oracle by the tests),
-    # per output: largest per-component relative error.
+    # One G'-column slab at its real shape: the share of one of 8 ranks in the
+    # reference's MPI decomposition (gppMPIOpenMP3.cpp:172, 255-271).
+    slab_cols = NGPOWN // 8
+    ctx.set_columns(0, slab_cols)
+    _, slab_ms, slab_st = time_form(ctx, args.form, max(1, args.steps // 2), args.warmup)
+    ctx.set_columns(0, 0)
+    slab = {"columns": [0, slab_cols], "of": NGPOWN, "form": args.form, "ms_per_step": slab_ms,
+            "updates_per_s": NB * slab_cols * NCOULS * NFREQ / (slab_ms * 1e-3), "gpu_stage_ms": slab_st}
+    # Agreement with the strict direct form (reference operation order, FP64),
+    # per output: largest per-component relative error.  The CPU leg below
+    # adds every form's error against the CPU oracle over the whole problem.
     bname = "direct-strict" if "direct-strict" in results else "direct"
     base = results[bname]
     for form, out in results.items():
         keys = ("achtemp", "asxtemp", "achstemp") + (("acht_n1_loc",) if out.get("acht_n1_loc") is not None else ())
-        forms[form]["rel_err_vs_" + bname.replace("-", "_")] = {k: rel_err(out[k], base[k]) for k in keys}
+        forms[form]["rel_err_vs_" + bname.replace("-", "_") + "_gpu"] = {k: rel_err(out[k], base[k]) for k in keys}
+        if form.startswith("factored-ozaki"):
+            forms[form]["rel_err_modulus_vs_" + bname.replace("-", "_") + "_gpu"] = {
+                k: rel_err_modulus(out[k], base[k]) for k in keys}
     for f in forms:
         if f.startswith("factored"):
             forms[f]["speedup_vs_direct"] = forms["direct"]["ms_per_step"] / forms[f]["ms_per_step"]
@@ -321,7 +379,7 @@ def main():
         flop = (NB - NV) * NGPOWN * NCOULS * NFREQ * FLOP_UNOCC
         achieved = flop / (kms * 1e-3) / 1e12
         tr = load_traffic("k_unocc") if (form == "direct" and args.workload == "c1") else None
-        return {"bound": "mfma", "pipe": "FP64 VALU (FP64 vector peak = FP64 matrix peak on MI355X)",
+        return {"bound": "valu", "pipe": "FP64 VALU (FP64 vector peak = FP64 matrix peak on MI355X)",
                 "kernel": kname, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS,
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
@@ -390,12 +448,26 @@ def main():
                    "form": args.form, "number_bands": NB, "nvband": NV, "ncouls": NCOULS, "ngpown": NGPOWN,
                    "nfreq": NFREQ, "updates_per_step": updates},
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
         "forms": forms,
+        "slab": slab,
+        "error_definition": ("largest relative error per real / imaginary component of each output (a component "
+                             "below 1e-6 of its complex modulus relative to the modulus); the Ozaki forms also "
+                             "report it relative to each output's modulus (rel_err_modulus), the bound their "
+                             "digit split guarantees"),
     }
-    if cpu:
-        line["gpu_over_cpu"] = head["updates_per_s"] / cpu["value"]
     ctx.close()
+    # The CPU leg, after the GPU work: the reference timed on the host, then
+    # the oracle over the whole benchmark problem as the checker.
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+        line["cpu_baseline"] = cpu
+        line["gpu_over_cpu"] = head["updates_per_s"] / cpu["value"]
+        if args.workload == "c1" and not args.no_cpu_parity:
+            par = cpu_parity(results)
+            line["cpu_parity"] = {k: v for k, v in par.items() if k != "rel_err_per_component"}
+            for form, e in par["rel_err_per_component"].items():
+                forms[form]["rel_err_vs_cpu_oracle"] = e
     print(json.dumps(line), flush=True)
 
 

@@ -20,8 +20,11 @@ __host__ __device__ __forceinline__ cplx mk(double re, double im) { return cplx{
 __host__ __device__ __forceinline__ cplx czero() { return cplx{0.0, 0.0}; }
 
 // The helpers below follow the reference's operation order and are compiled
-// without FMA contraction (the pragma sits in each body because it does not
-// reach inlined callees), so each rounds exactly as the x86-64 reference does.
+// without FMA contraction, so each rounds exactly as the x86-64 reference
+// does.  The pragma sits in every body, the multiplications and additions
+// alone included: it does not reach inlined callees, and an unmarked multiply
+// of one helper feeding an unmarked add of another (delw = x * rden, then
+// 0.5 + delw) is otherwise fused into one FMA.
 
 // GPPComplex_product, Complex.h:167-174.
 __device__ __forceinline__ cplx cmul(cplx a, cplx b)
@@ -30,7 +33,11 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b)
     return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 // GPPComplex_conj, Complex.h:153-161.
-__device__ __forceinline__ cplx cconj(cplx a) { return cplx{a.re, -1 * a.im}; }
+__device__ __forceinline__ cplx cconj(cplx a)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re, -1 * a.im};
+}
 // GPPComplex_square, Complex.h:141-148.
 __device__ __forceinline__ cplx csquare(cplx a)
 {
@@ -38,13 +45,29 @@ __device__ __forceinline__ cplx csquare(cplx a)
     return cplx{a.re * a.re - a.im * a.im, 2 * a.re * a.im};
 }
 // GPPComplex_mult(a, b), Complex.h:200-205.
-__device__ __forceinline__ cplx cscale(cplx a, double b) { return cplx{a.re * b, a.im * b}; }
+__device__ __forceinline__ cplx cscale(cplx a, double b)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re * b, a.im * b};
+}
 // GPPComplex_mult(a, b, c), Complex.h:190-195: ((re*b)*c, (im*b)*c).
-__device__ __forceinline__ cplx cscale2(cplx a, double b, double c) { return cplx{a.re * b * c, a.im * b * c}; }
+__device__ __forceinline__ cplx cscale2(cplx a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re * b * c, a.im * b * c};
+}
 // doubleMinusGPPComplex, Complex.h:234-237.
-__device__ __forceinline__ cplx cdminus(double a, cplx z) { return cplx{a - z.re, 0 - z.im}; }
+__device__ __forceinline__ cplx cdminus(double a, cplx z)
+{
+#pragma clang fp contract(off)
+    return cplx{a - z.re, 0 - z.im};
+}
 // doublePlusGPPComplex, Complex.h:239-242.
-__device__ __forceinline__ cplx cdplus(double a, cplx z) { return cplx{a + z.re, 0 + z.im}; }
+__device__ __forceinline__ cplx cdplus(double a, cplx z)
+{
+#pragma clang fp contract(off)
+    return cplx{a + z.re, 0 + z.im};
+}
 // GPPComplex_abs, Complex.h:179-185 (with sqrt).
 __device__ __forceinline__ double cabs_(cplx a)
 {
@@ -54,10 +77,15 @@ __device__ __forceinline__ double cabs_(cplx a)
 // operator+=, Complex.h:51-56: dst = src + dst.
 __device__ __forceinline__ void cacc(cplx &dst, cplx src)
 {
+#pragma clang fp contract(off)
     dst.re = src.re + dst.re;
     dst.im = src.im + dst.im;
 }
-__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b)
+{
+#pragma clang fp contract(off)
+    return cplx{a.re + b.re, a.im + b.im};
+}
 
 // 1/d for d > 0 to within about one ulp: the hardware estimate refined by two
 // Newton-Raphson steps (each at least doubles the correct bits).  d = 0 gives
@@ -167,9 +195,9 @@ __device__ __forceinline__ void occ_pole_terms_uncut(double wxt, double wx2, dou
 }
 
 // |ssx| of the reference's branch 2 (gppComplex.cpp:44-50) for one element,
-// in its operation order: the rare near-cutoff path of the factored reduce,
-// kept out of line so that it does not add to the reduce loop's registers.
-__device__ __attribute__((noinline)) double occ_branch2_abs_ssx(double wxt, cplx wt, cplx ie)
+// in its operation order: the rare near-cutoff path of the factored reduce
+// (only in its negative-frequency kernels).
+__device__ __forceinline__ double occ_branch2_abs_ssx(double wxt, cplx wt, cplx ie)
 {
 #pragma clang fp contract(off)
     const cplx wt2 = csquare(wt);                                          // :117

@@ -415,11 +415,33 @@ void oracle_dd_mul_d(double hi, double lo, double b, double *out)
     out[0] = r.hi; out[1] = r.lo;
 }
 
+int oracle_sigma_dd2(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
+                     const int *inv_igp_index, const double *wx_array, const double *vcoul,
+                     const double *aqsmtemp, const double *aqsntemp,
+                     const double *I_eps_array, const double *wtilde_array,
+                     double *achtemp_out, double *asxtemp_out, double *acht_n1_loc_out,
+                     double *achstemp_out, int nthreads);
+
 int oracle_sigma_dd(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
                     const int *inv_igp_index, const double *wx_array, const double *vcoul,
                     const double *aqsmtemp, const double *aqsntemp,
                     const double *I_eps_array, const double *wtilde_array,
                     double *achtemp_out, double *asxtemp_out, int nthreads)
+{
+    return oracle_sigma_dd2(nb, nv, ncouls, ngpown, nfreq, occ, inv_igp_index, wx_array, vcoul, aqsmtemp,
+                            aqsntemp, I_eps_array, wtilde_array, achtemp_out, asxtemp_out, NULL, NULL, nthreads);
+}
+
+/* The compensated loop with every output: acht_n1_loc[nb] (gppComplex.cpp:381,
+ * each band's frequency-acht_iw terms summed in double-double) and achstemp
+ * (reduce_achstemp, :66-101, its per-column terms summed in double-double);
+ * either may be NULL. */
+int oracle_sigma_dd2(int nb, int nv, int ncouls, int ngpown, int nfreq, double occ,
+                     const int *inv_igp_index, const double *wx_array, const double *vcoul,
+                     const double *aqsmtemp, const double *aqsntemp,
+                     const double *I_eps_array, const double *wtilde_array,
+                     double *achtemp_out, double *asxtemp_out, double *acht_n1_loc_out,
+                     double *achstemp_out, int nthreads)
 {
     int rc = oc_check(nb, nv, ncouls, ngpown, nfreq, inv_igp_index);
     if (rc) return rc;
@@ -472,6 +494,36 @@ int oracle_sigma_dd(int nb, int nv, int ncouls, int ngpown, int nfreq, double oc
         achtemp_out[2 * iw + 1] = a.im.hi + a.im.lo;
         asxtemp_out[2 * iw] = s.re.hi + s.re.lo;
         asxtemp_out[2 * iw + 1] = s.im.hi + s.im.lo;
+    }
+    if (acht_n1_loc_out) {
+        const int acht_iw = oc_acht_iw(nfreq);
+        for (int n1 = 0; n1 < nb; ++n1) {
+            oc_ddc a;
+            memset(&a, 0, sizeof a);
+            for (int j = 0; j < ngpown; ++j) oc_ddc_acc_dd(&a, ach_slots[((long)n1 * ngpown + j) * nfreq + acht_iw]);
+            acht_n1_loc_out[2 * n1] = a.re.hi + a.re.lo;
+            acht_n1_loc_out[2 * n1 + 1] = a.im.hi + a.im.lo;
+        }
+    }
+    if (achstemp_out) {  /* reduce_achstemp's terms, :84-99, in double-double */
+        oc_ddc a;
+        memset(&a, 0, sizeof a);
+        for (int n1 = 0; n1 < nb; ++n1)
+            for (int j = 0; j < ngpown; ++j) {
+                const int igp = oc_igp(inv_igp_index, j, ncouls);
+                const long brow = (long)n1 * ncouls;
+                const oc_cplx mygpvar1 = oc_conj(oc_load(aqsmtemp, brow + igp));
+                const oc_cplx schs = oc_neg(oc_load(I_eps_array, (long)j * ncouls + igp));
+                const oc_cplx m = oc_product(oc_load(aqsntemp, brow + igp), mygpvar1);
+                if (oc_abs(schs) > OC_TO1) {
+                    oc_ddc t;
+                    memset(&t, 0, sizeof t);
+                    oc_ddc_acc(&t, oc_product(m, schs));
+                    oc_ddc_acc_dd(&a, oc_ddc_mul_d(t, vcoul[igp] * (double)0.5));
+                }
+            }
+        achstemp_out[0] = a.re.hi + a.re.lo;
+        achstemp_out[1] = a.im.hi + a.im.lo;
     }
     free(ach_slots); free(asx_slots);
     return 0;

@@ -44,6 +44,8 @@ def load_oracle() -> ctypes.CDLL:
         lib.oracle_sigma_wxb.restype = ctypes.c_int
         lib.oracle_sigma_dd.argtypes = _sigma_argtypes([vp, vp, ctypes.c_int])
         lib.oracle_sigma_dd.restype = ctypes.c_int
+        lib.oracle_sigma_dd2.argtypes = _sigma_argtypes([vp, vp, vp, vp, ctypes.c_int])
+        lib.oracle_sigma_dd2.restype = ctypes.c_int
         lib.oracle_pole_terms.argtypes = [ctypes.c_double, vp, vp, vp, vp]
         lib.oracle_pole_terms.restype = ctypes.c_int
         lib.oracle_max_threads.restype = ctypes.c_int
@@ -117,20 +119,31 @@ def sigma(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: i
     return out
 
 
-def sigma_dd(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: int = 1):
-    """Compensated (double-double accumulated) achtemp, asxtemp."""
+def sigma_dd(arrs: Dict[str, np.ndarray], nvband: int, occ: float = 1.0, threads: int = 1,
+             all_outputs: bool = False):
+    """Compensated (double-double accumulated) achtemp, asxtemp and, with
+    all_outputs, acht_n1_loc and achstemp: the sums of the reference's own
+    rounded terms, without the error of its left-to-right sums."""
     if np.ndim(arrs["wx_array"]) != 1:
         raise ValueError("sigma_dd takes one wx_array for all bands")
     ins = _prep(arrs)
     sizes = _sizes(arrs, nvband, occ)
-    nfreq = sizes[4]
+    nb, nfreq = sizes[0], sizes[4]
     ach = np.zeros(nfreq, np.complex128)
     asx = np.zeros(nfreq, np.complex128)
-    rc = load_oracle().oracle_sigma_dd(*sizes, *[a.ctypes.data for a in ins], ach.ctypes.data,
-                                       asx.ctypes.data, threads)
+    if not all_outputs:
+        rc = load_oracle().oracle_sigma_dd(*sizes, *[a.ctypes.data for a in ins], ach.ctypes.data,
+                                           asx.ctypes.data, threads)
+        if rc != 0:
+            raise ValueError(f"oracle rejected the inputs (code {rc})")
+        return {"achtemp": ach, "asxtemp": asx}
+    n1 = np.zeros(nb, np.complex128)
+    achs = np.zeros(1, np.complex128)
+    rc = load_oracle().oracle_sigma_dd2(*sizes, *[a.ctypes.data for a in ins], ach.ctypes.data, asx.ctypes.data,
+                                        n1.ctypes.data, achs.ctypes.data, threads)
     if rc != 0:
         raise ValueError(f"oracle rejected the inputs (code {rc})")
-    return {"achtemp": ach, "asxtemp": asx}
+    return {"achtemp": ach, "asxtemp": asx, "acht_n1_loc": n1, "achstemp": achs[0]}
 
 
 def pole_terms(wxt: float, wtilde: complex, ieps: complex, use_ref: bool = False):

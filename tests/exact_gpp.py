@@ -88,6 +88,7 @@ def sigma_exact(arrs, nv, occ=1.0):
     ach_mag = [0.0] * nfreq
     asx_mag = [0.0] * nfreq
     n1loc = [zero] * nb
+    n1loc_mag = [0.0] * nb
     achs = zero
     achs_mag = 0.0
     for n1 in range(nb):
@@ -126,12 +127,14 @@ def sigma_exact(arrs, nv, occ=1.0):
                 ach_mag[iw] += sch_mag * float(vcg) / 2
                 if iw == acht_iw:
                     n1loc[n1] = _add(n1loc[n1], ta)
+                    n1loc_mag[n1] += sch_mag * float(vcg) / 2
                 if n1 < nv:
                     tx = _scale(ssxt, occ * vcg)
                     asx[iw] = _add(asx[iw], tx)
                     asx_mag[iw] += ssx_mag * float(occ * vcg)
     return {"achtemp": ach, "asxtemp": asx, "acht_n1_loc": n1loc, "achstemp": achs,
-            "achtemp_mag": ach_mag, "asxtemp_mag": asx_mag, "achstemp_mag": achs_mag}
+            "achtemp_mag": ach_mag, "asxtemp_mag": asx_mag, "achstemp_mag": achs_mag,
+            "acht_n1_loc_mag": n1loc_mag}
 
 
 def to_complex(p):

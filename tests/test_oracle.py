@@ -353,8 +353,11 @@ def _dyadic_cancelling_inputs():
 def test_compensated_oracle_exact_on_dyadic_cancellation(oracle):
     a = _dyadic_cancelling_inputs()
     ex = exact_gpp.sigma_exact(a, nv=1)
-    d = oracle.sigma_dd(a, nvband=1)
+    d = oracle.sigma_dd(a, nvband=1, all_outputs=True)
     s = oracle.sigma(a, nvband=1)
+    # acht_n1_loc (frequency 1 of 2 here: wx = 3, exact terms) of every band
+    for n1 in range(3):
+        assert d["acht_n1_loc"][n1] == exact_gpp.to_complex(ex["acht_n1_loc"][n1]), n1
     worst_seq = 0.0
     checked = 0
     for k, iws in (("achtemp", (0, 1)), ("asxtemp", (0,))):  # asxtemp at wx=3 has inexact 1/81 terms
@@ -387,12 +390,19 @@ def test_compensated_oracle_vs_exact_rational(oracle, case):
         a["I_eps_array"][:, 6:] = a["I_eps_array"][:, :6]
         a["wtilde_array"][:, 6:] = a["wtilde_array"][:, :6]
     ex = exact_gpp.sigma_exact(a, nv)
-    d = oracle.sigma_dd(a, nvband=nv)
+    d = oracle.sigma_dd(a, nvband=nv, all_outputs=True)
+    d2 = oracle.sigma_dd(a, nvband=nv, threads=4)
     for k in ("achtemp", "asxtemp"):
+        np.testing.assert_array_equal(d[k], d2[k])
         for iw in range(len(a["wx_array"])):
             e = exact_gpp.to_complex(ex[k][iw])
             scale = ex[k + "_mag"][iw] + 1e-300
             assert abs(d[k][iw] - e) <= 2e-16 * scale, (case, k, iw, d[k][iw], e)
+    for n1 in range(a["aqsmtemp"].shape[0]):
+        e = exact_gpp.to_complex(ex["acht_n1_loc"][n1])
+        assert abs(d["acht_n1_loc"][n1] - e) <= 2e-16 * (ex["acht_n1_loc_mag"][n1] + 1e-300), (case, n1)
+    e = exact_gpp.to_complex(ex["achstemp"])
+    assert abs(d["achstemp"] - e) <= 2e-16 * (ex["achstemp_mag"] + 1e-300)
 
 
 # ---- the compiled reference --------------------------------------------------
