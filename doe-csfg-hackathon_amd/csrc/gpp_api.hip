@@ -1490,3 +1490,91 @@ gpp_status d_noFlagOCCSolver(int number_bands, int nvband, int ngpown, int ncoul
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The launcher that allocates nothing: d_noFlagOCCSolver with its scratch --
+// the index flag, the results and the direct form's partial sums -- in one
+// file-scope device array instead of the grow-only per-device workspace.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;  // 16 MiB per device
+constexpr int kNoAllocMaxDevices = 64;
+
+}  // namespace
+
+// The scratch array.  In host code this name is only a stand-in for the
+// symbol: its device address (one per device) comes from hipGetSymbolAddress
+// and is the only handle host code passes on.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};  // device address of scratch on each device (guarded by g_compat_mu)
+
+// Device address of the scratch array on the current device `dev`, looked up
+// on the first call there.
+gpp_status noalloc_scratch(int dev, char **base)
+{
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(GPP_ERR_NO_DEVICE, "device %d beyond the %d the no-alloc launcher serves", dev, kNoAllocMaxDevices);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        HIP_OR_FAIL(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(GPP_ERR_LAUNCH, "hipGetSymbolAddress(scratch) returned NULL");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *base = static_cast<char *>(g_noalloc_ptr[dev]);
+    return GPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gpp_status d_noFlagOCCSolver_noalloc(int number_bands, int nvband, int ngpown, int ncouls, int *inv_igp_index,
+                                     double *wx_array, gpp_complex *wtilde_array, gpp_complex *aqsmtemp,
+                                     gpp_complex *aqsntemp, gpp_complex *I_eps_array, double *vcoul,
+                                     double *achtemp_re, double *achtemp_im, gpp_complex *acht_n1_loc)
+{
+    if (number_bands < 1 || nvband < 0 || nvband > number_bands || ngpown < 1 || ncouls < 1)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "invalid sizes number_bands=%d nvband=%d ngpown=%d ncouls=%d",
+                    number_bands, nvband, ngpown, ncouls);
+    if (nvband == number_bands) return GPP_OK;
+    if (!inv_igp_index || !wx_array || !wtilde_array || !aqsmtemp || !aqsntemp || !I_eps_array || !vcoul ||
+        !achtemp_re || !achtemp_im)
+        return fail(GPP_ERR_INVALID_ARGUMENT, "NULL device array");
+    const gpp::DevProblem dp = compat_problem(number_bands, nvband, ncouls, ngpown, inv_igp_index, wx_array, vcoul,
+                                              aqsmtemp, aqsntemp, I_eps_array, wtilde_array, 1.0);
+    // scratch: [index flag: 256 B][results][direct-form partial sums]
+    gpp::DevResults res;
+    res.nfreq = dp.nfreq;
+    res.nb = dp.nb;
+    const size_t res_bytes = (sizeof(cplx) * res.count() + 255) & ~(size_t)255;
+    const size_t need = 256 + res_bytes + gpp::direct_workspace_bytes(dp);
+    if (need > kNoAllocScratchBytes)
+        return fail(GPP_ERR_INVALID_ARGUMENT,
+                    "number_bands=%d ngpown=%d needs %zu bytes of scratch; d_noFlagOCCSolver_noalloc holds %zu "
+                    "(use d_noFlagOCCSolver)",
+                    number_bands, ngpown, need, kNoAllocScratchBytes);
+    std::lock_guard<std::mutex> lock(g_compat_mu);  // calls share the device's scratch: one at a time
+    int dev = 0;
+    HIP_OR_FAIL(hipGetDevice(&dev), "hipGetDevice");
+    char *base = nullptr;
+    gpp_status st = noalloc_scratch(dev, &base);
+    if (st) return st;
+    hipStream_t s = nullptr;  // null stream, as the reference launches
+    int *flag = reinterpret_cast<int *>(base);
+    HIP_OR_FAIL(gpp::launch_zero_ints(flag, 1, s), "zero flag");  // the last call's flag stays in scratch
+    HIP_OR_FAIL(gpp::launch_check_index(dp.idx, dp.ngpown, flag, s), "check_index");
+    int bad = 0;
+    HIP_OR_FAIL(hipMemcpy(&bad, flag, sizeof(int), hipMemcpyDeviceToHost), "D2H flag");
+    if (bad) return fail(GPP_ERR_BAD_INDEX, "%d negative inv_igp_index entries", bad);
+    res.base = reinterpret_cast<cplx *>(base + 256);
+    HIP_OR_FAIL(gpp::run_direct_part(dp, 1, base + 256 + res_bytes, res, s), "direct launch");
+    HIP_OR_FAIL(gpp::launch_accumulate_compat(res, 1, dp.nv, achtemp_re, achtemp_im, nullptr,
+                                              reinterpret_cast<cplx *>(acht_n1_loc), nullptr, nullptr, s),
+                "accumulate launch");
+    return GPP_OK;
+}
+
+}  // extern "C"

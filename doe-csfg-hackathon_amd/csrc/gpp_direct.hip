@@ -404,6 +404,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_final(DirectArgs a, c
     if (threadIdx.x == 0) out[2 * nf + a.nb] = h;
 }
 
+__global__ void k_zero_ints(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 __global__ void k_check_index(const int32_t *idx, int n, int *count)
 {
     int bad = 0;
@@ -546,6 +551,13 @@ hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hi
     if (ngpown <= 0) return hipSuccess;
     const int blocks = (ngpown + 255) / 256 < 1024 ? (ngpown + 255) / 256 : 1024;
     hipLaunchKernelGGL(k_check_index, dim3(blocks), dim3(256), 0, s, d_idx, ngpown, d_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int *d_p, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_ints, dim3((n + 255) / 256 < 64 ? (n + 255) / 256 : 64), dim3(256), 0, s, d_p, n);
     return hipGetLastError();
 }
 

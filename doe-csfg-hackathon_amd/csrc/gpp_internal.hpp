@@ -102,6 +102,9 @@ hipError_t run_direct_part(const DevProblem &p, int which, void *work, const Dev
 // rows Bg[n1][j] = aqsm[n1][igp(j)] [nb][ngpown].
 hipError_t launch_achstemp(const DevProblem &p, cplx *part_achs, cplx *bg, hipStream_t s);
 
+// Zeroes n ints with a fill kernel (no memset node when a caller captures).
+hipError_t launch_zero_ints(int *d_p, int n, hipStream_t s);
+
 // Counts negative inv_igp_index entries on the device (result in *d_count).
 hipError_t launch_check_index(const int32_t *d_idx, int ngpown, int *d_count, hipStream_t s);
 

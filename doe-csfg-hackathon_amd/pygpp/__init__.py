@@ -92,6 +92,7 @@ EXPORTED_SYMBOLS = (
     "d_till_nvband",
     "d_reduce_achstemp",
     "d_noFlagOCCSolver",
+    "d_noFlagOCCSolver_noalloc",
 )
 
 
@@ -194,13 +195,14 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.d_till_nvband.argtypes = [i32, i32, i32] + [vp] * 11 + [dbl]
     lib.d_reduce_achstemp.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, i32, vp]
     lib.d_noFlagOCCSolver.argtypes = [i32, i32, i32, i32] + [vp] * 10
+    lib.d_noFlagOCCSolver_noalloc.argtypes = [i32, i32, i32, i32] + [vp] * 10
     for name in ("gpp_ctx_create", "gpp_ctx_destroy", "gpp_upload", "gpp_bind_device", "gpp_ctx_set_stream",
                  "gpp_ctx_set_columns", "gpp_sigma_async", "gpp_ctx_wait",
                  "gpp_fill_deterministic", "gpp_fill_seeded", "gpp_generate_seeded", "gpp_sigma",
                  "gpp_ctx_last_timing", "gpp_ctx_set_option", "gpp_ctx_get_option", "gpp_ctx_autotune",
                  "gpp_batch_create", "gpp_batch_create_seeded", "gpp_batch_sigma", "gpp_batch_autotune",
                  "gpp_batch_use_graph", "gpp_batch_last_timing", "gpp_batch_last_stages", "gpp_batch_destroy",
-                 "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver"):
+                 "d_till_nvband", "d_reduce_achstemp", "d_noFlagOCCSolver", "d_noFlagOCCSolver_noalloc"):
         getattr(lib, name).restype = i32
     _lib = lib
     return lib

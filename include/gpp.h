@@ -485,6 +485,28 @@ gpp_status d_noFlagOCCSolver(int number_bands, int nvband, int ngpown, int ncoul
                              gpp_complex *I_eps_array, double *vcoul, double *achtemp_re,
                              double *achtemp_im, gpp_complex *acht_n1_loc);
 
+/*
+ * d_noFlagOCCSolver without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (harnesses
+ * that forbid hipMalloc, tight serving loops, code that must not grow the
+ * device heap).  Same parameters, meaning, accumulation and status as
+ * d_noFlagOCCSolver; its scratch (index flag, results and partial sums, about
+ * 16 B x number_bands x ceil(ngpown/4) x 3 plus small tables) lives in a
+ * 16 MiB device array of the library, one per device, held until the process
+ * ends -- enough for number_bands x ngpown up to about 512 x 1638 or 1024 x
+ * 1000.  Larger problems are refused with GPP_ERR_INVALID_ARGUMENT.  Like
+ * d_noFlagOCCSolver it synchronises once to validate inv_igp_index and then
+ * enqueues on the null stream.  Calls on one device share the array (each
+ * overwrites what the last left there): the library serialises its own calls,
+ * so calls that overlap on one device -- from several host threads, or before
+ * the previous call's work has run -- keep their order on the null stream.
+ */
+gpp_status d_noFlagOCCSolver_noalloc(int number_bands, int nvband, int ngpown, int ncouls,
+                                     int *inv_igp_index, double *wx_array, gpp_complex *wtilde_array,
+                                     gpp_complex *aqsmtemp, gpp_complex *aqsntemp,
+                                     gpp_complex *I_eps_array, double *vcoul, double *achtemp_re,
+                                     double *achtemp_im, gpp_complex *acht_n1_loc);
+
 #ifdef __cplusplus
 }
 #endif
