@@ -95,8 +95,8 @@ DirectArgs make_args(const DevProblem &p, void *work)
 //   a * ((wtilde * conj(wdiff) * rden) * I_eps),  rden = 1/|wdiff|^2
 // is evaluated as ((a * (wtilde * I_eps)) * conj(wdiff)) * rden, the same
 // product re-associated; wdiff = wx - wtilde and the reciprocal are formed
-// per update as in gppComplex.cpp:139-143, the reciprocal with one Newton
-// step (about 2^-45 relative, one-signed: a sum moves by at most ~1.4e-14).
+// per update as in gppComplex.cpp:139-143, the reciprocal refined to third
+// order (rcp_2o: about one rounding, unbiased, as the reference's quotient).
 // wtie = wtilde * I_eps, formed once per (column, G-vector) when staged.
 template <int FC>
 __device__ __forceinline__ void unocc_update_fast(cplx (&acc)[FC], cplx cm, cplx an, cplx wt,
@@ -108,7 +108,7 @@ __device__ __forceinline__ void unocc_update_fast(cplx (&acc)[FC], cplx cm, cplx
     for (int f = 0; f < FC; ++f) {
         const double wdr = wxv[f] - wt.re;   // wdiff = (wx - wt.re, -wt.im)
         const double d = wdr * wdr + wt.im * wt.im;
-        const double r = rcp_1nr(d);
+        const double r = rcp_2o(d);
         // q * conj(wdiff) = q * (wdr, wt.im)
         const double tr = q.re * wdr - q.im * wt.im;
         const double ti = q.im * wdr + q.re * wt.im;

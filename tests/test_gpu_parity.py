@@ -37,17 +37,23 @@ def assert_close(out, ref, tol=TOL, keys=("achtemp", "asxtemp", "acht_n1_loc", "
 
 
 def assert_close_conditioned(out, seq, dd, tol=TOL, keys=("achtemp", "asxtemp")):
-    """For ill-conditioned sums: within tol of the exactly summed reference
-    terms (dd, the compensated oracle), and within tol of the sequential
-    reference plus that reference's own summation error |seq - dd|."""
+    """For ill-conditioned sums, per real / imaginary component: the GPU value
+    lies within tol of the exact sum of the reference's rounded terms (dd, the
+    compensated oracle) -- or no farther from it than the reference's own
+    left-to-right sum (seq) is, |seq - dd|, plus tol.  Where cancellation makes
+    a component a tiny fraction of its output, every form whose terms round
+    differently from the reference's carries an error of the terms' size, not
+    the component's, exactly as the reference's sequential sum does."""
     for k in keys:
         g = np.atleast_1d(np.asarray(out[k], np.complex128))
         s_ = np.atleast_1d(np.asarray(seq[k], np.complex128))
         d = np.atleast_1d(np.asarray(dd[k], np.complex128))
-        assert comp_rel_err(g, d) <= tol, f"{k}: vs compensated oracle {comp_rel_err(g, d):.3e}"
         for gc, sc, dc in ((g.real, s_.real, d.real), (g.imag, s_.imag, d.imag)):
-            bound = tol * np.maximum(np.abs(sc), 1e-6 * np.abs(d)) + np.abs(sc - dc)
-            assert np.all(np.abs(gc - sc) <= bound), f"{k}: {np.abs(gc - sc)} > {bound}"
+            own = np.abs(sc - dc)  # the reference's own summation error
+            scale = np.maximum(np.abs(dc), 1e-6 * np.abs(d))
+            assert np.all(np.abs(gc - dc) <= tol * scale + own), \
+                f"{k}: vs compensated oracle {np.abs(gc - dc) / scale} > {tol} + {own / scale}"
+            assert np.all(np.abs(gc - sc) <= tol * scale + 2 * own), f"{k}: vs sequential oracle"
 
 
 def problem_of(arrs, nv):

@@ -10,8 +10,8 @@ reference on every fixture):
   components by up to a few 1e-12 (the direct-strict form, whose terms are
   the reference's bit for bit, differs from it by that much); there the
   direct forms are held to 1e-12 of the compensated oracle (the exact sum of
-  the reference's terms) and to 1e-12 plus the reference's own summation
-  error of the sequential oracle (assert_close_conditioned);
+  the reference's terms), or to no more than the reference's own distance
+  from it (assert_close_conditioned);
 * the batched workload (64 instances, nb up to 1024, ncouls up to 16384,
   nfreq up to 32) through one batch with the FP64 and the INT8 Ozaki Gram
   product: every instance against the direct form run on that instance, the
