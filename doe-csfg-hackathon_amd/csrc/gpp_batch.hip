@@ -144,8 +144,9 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 }
 
 // Reduce items of instances whose occupied band count is NV (0: any count);
-// NEG: instances with a negative frequency (generic count).  Occupancy as the
-// single-problem reduce: two waves per SIMD at FC = 8.
+// NEG: instances with a negative frequency (generic count).  Two waves per
+// SIMD above three frequencies: three spill 10-22 VGPRs in the batch classes
+// and time the same on the 64-instance batch.
 template <int FC, int NV, bool NEG = false>
 __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
                                                               const RedItem *__restrict__ items, int nitems,

@@ -123,6 +123,18 @@ __device__ __forceinline__ double rcp_2o(double d)
     return __builtin_fma(r, __builtin_fma(e, e, e), r);
 }
 
+// A wave-uniform value moved into scalar registers (readfirstlane of each
+// 32-bit half): it then costs no VGPRs where it is live.  Only for values
+// every lane of the wave holds alike.
+__device__ __forceinline__ double uniform(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ cplx uniform(cplx x) { return cplx{uniform(x.re), uniform(x.im)}; }
+
 // Product a*b with FMA contraction allowed (fast paths only).
 __device__ __forceinline__ cplx cmul_fast(cplx a, cplx b)
 {

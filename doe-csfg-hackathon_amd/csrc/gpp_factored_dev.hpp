@@ -198,10 +198,11 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     // conj(Bg[n1][j]) of the occupied bands, loaded once for the column (the
     // optional p2/br1 stores in the loop would otherwise keep the compiler
     // from hoisting these loads out of it).
+    // Wave-uniform (the column's), so kept in scalar registers.
     cplx bgc[NV > 0 ? NV : 1];
     if constexpr (NV > 0) {
 #pragma unroll
-        for (int n1 = 0; n1 < NV; ++n1) bgc[n1] = cconj(a.bg[(size_t)n1 * a.ngpown + j]);
+        for (int n1 = 0; n1 < NV; ++n1) bgc[n1] = uniform(cconj(a.bg[(size_t)n1 * a.ngpown + j]));
     }
     // Frequencies in passes of FC, the last one FCT wide when at most FCT
     // remain; the passes after the first re-read this block's G-vector range,
@@ -212,9 +213,10 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         double wxv[F], c1[F];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
 #pragma unroll
         for (int f = 0; f < F; ++f) {
-            wxv[f] = (iw0 + f < iw_end) ? a.wx[iw0 + f] : 1.0;
+            // wave-uniform, so kept in scalar registers
+            wxv[f] = uniform((iw0 + f < iw_end) ? a.wx[iw0 + f] : 1.0);
             const double wx2 = wxv[f] * wxv[f];
-            c1[f] = wx2 * (1.00 / (wx2 * wx2));
+            c1[f] = uniform(wx2 * (1.00 / (wx2 * wx2)));
         }
         cplx sach[F], sasx[F];
 #pragma unroll
