@@ -344,8 +344,15 @@ def main():
     # The factored forms compute acht_n1_loc only on request (a second product
     # of the Gram product's size); "factored-f64+acht" is that run, with every
     # output of the reference's band loop.
-    for form in run_forms + ["factored-f64+acht"]:
+    # "factored-f64-fused" is factored-f64 with the pole reduce in the Gram
+    # epilogue (GPP_OPT_FUSED_REDUCE = 1, at most three frequencies): no G'xG
+    # workspace, slower where that fits.
+    extra = ["factored-f64+acht"] + (["factored-f64-fused"] if NFREQ <= 3 else [])
+    for form in run_forms + extra:
         name, acht = (form[:-5], True) if form.endswith("+acht") else (form, None)
+        fused = name.endswith("-fused")
+        if fused:
+            name = name[:-len("-fused")]
         # the direct forms (50 ms per step at c1, 3 s at c3) run at their
         # sustained clock from the first step: a few steps and one warmup
         if form == args.form:
@@ -354,7 +361,9 @@ def main():
             steps, warm = max(1, min(args.steps // 2, 5 if args.workload == "c1" else 2)), min(args.warmup, 1)
         else:
             steps, warm = max(1, args.steps // 2), args.warmup
+        ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, 1 if fused else 0)
         out, ms, stages = time_form(ctx, name, steps, warm, acht)
+        ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, 0)
         results[form] = out
         forms[form] = {"ms_per_step": ms, "updates_per_s": updates / (ms * 1e-3), "gpu_stage_ms": stages}
     head = forms[args.form]
@@ -398,10 +407,13 @@ def main():
         gkey = [k for k in st if k.startswith("factored_gram")][0]
         gms = st[gkey]
         K = NB - NV
-        if form == "factored-f64":
+        fused = gkey == "factored_gram_pole"
+        if form.startswith("factored-f64"):
             flop = F64_FLOP_PER_CMAC * NGPOWN * NCOULS * K
             peak, pname = PEAK_FP64_TFLOPS, "v_mfma_f64_16x16x4, 3M: 6 flop per complex multiply-add"
-            tr = load_traffic("k_gram_f64") if args.workload == "c1" else None
+            if fused:
+                pname += "; the pole reduce of its elements in the epilogue, timed with it"
+            tr = load_traffic("k_gram_pole" if fused else "k_gram_f64") if args.workload == "c1" else None
         else:
             # executed Ozaki work: 21 digit pairs x 4 real products, split included in the stage time
             flop = 21 * 4 * 2 * NGPOWN * NCOULS * K
@@ -411,19 +423,27 @@ def main():
             pname = ("v_mfma_i32_32x32x32_i8" if i8 else "v_mfma_f32_32x32x16_f16") + ", digit splits included"
             tr = None
         achieved = flop / (gms * 1e-3) / 1e12
+        extra = {"zgemm_equivalent_tflops": achieved * 8 / F64_FLOP_PER_CMAC} if form.startswith("factored-f64") else {}
+        line = {**extra, "bound": "mfma", "kernel": f"{gkey} ({pname})", "achieved": achieved,
+                "peak": peak, "unit": "TFLOP/s" if not form.endswith("i8") else "TOP/s", "frac": achieved / peak,
+                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+                "algorithmic_flop_per_launch": flop, "avg_launch_ms": gms}
+        if fused:
+            # the Gram kernel also streams wtilde and I_eps for the pole terms
+            cms = st["factored_pole_cols"]
+            line["fused_epilogue"] = {"algorithmic_bytes": 2 * 16 * NGPOWN * NCOULS,
+                                      "note": "wtilde and I_eps read in the Gram epilogue; Mu stays in registers",
+                                      "column_sums_kernel": "factored_pole_cols", "column_sums_ms": cms}
+            return line
         red_ms = st["factored_pole_reduce"]
         red_bytes = 3 * 16 * NGPOWN * NCOULS  # wtilde, I_eps, Mu
         tr2 = load_traffic("k_pole_reduce") if args.workload == "c1" else None
-        extra = {"zgemm_equivalent_tflops": achieved * 8 / F64_FLOP_PER_CMAC} if form == "factored-f64" else {}
-        return {**extra, "bound": "mfma", "kernel": f"{gkey} ({pname})", "achieved": achieved,
-                "peak": peak, "unit": "TFLOP/s" if not form.endswith("i8") else "TOP/s", "frac": achieved / peak,
-                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
-                "algorithmic_flop_per_launch": flop, "avg_launch_ms": gms,
-                "second_kernel": {"bound": "hbm", "kernel": "factored_pole_reduce", "avg_launch_ms": red_ms,
-                                  "algorithmic_bytes": red_bytes,
-                                  "achieved": red_bytes / (red_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
-                                  "unit": "GB/s", "frac": red_bytes / (red_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                                  "traffic": (tr2["fetch_bytes"] + tr2["write_bytes"]) if tr2 else None}}
+        line["second_kernel"] = {"bound": "hbm", "kernel": "factored_pole_reduce", "avg_launch_ms": red_ms,
+                                 "algorithmic_bytes": red_bytes,
+                                 "achieved": red_bytes / (red_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                                 "unit": "GB/s", "frac": red_bytes / (red_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                 "traffic": (tr2["fetch_bytes"] + tr2["write_bytes"]) if tr2 else None}
+        return line
 
     roofline = direct_roofline(args.form) if args.form.startswith("direct") else factored_roofline(args.form)
     for form in forms:

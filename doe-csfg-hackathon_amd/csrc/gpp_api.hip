@@ -191,6 +191,7 @@ struct gpp_ctx {
     int red_chunk = 8192;
     int red_fc = 8;
     int red_fpb = 0;
+    int fuse_reduce = 0;
     int num_cus = 256;
     int max_buckets = 32;
     // band-dependent frequencies (gpp_set_band_frequencies)
@@ -255,6 +256,7 @@ gpp::GramKind gram_kind(const gpp_ctx *ctx, gpp_form form)
     k.red_chunk = ctx->red_chunk;
     k.red_fc = ctx->red_fc;
     k.red_fpb = ctx->red_fpb;
+    k.fuse = ctx->fuse_reduce != 0;
     return k;
 }
 
@@ -861,6 +863,10 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
             return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs per block must be 0..4096, got %d", value);
         ctx->red_fpb = value;
         return GPP_OK;
+    case GPP_OPT_FUSED_REDUCE:
+        if (value != 0 && value != 1) return fail(GPP_ERR_INVALID_ARGUMENT, "fused reduce must be 0 or 1, got %d", value);
+        ctx->fuse_reduce = value;
+        return GPP_OK;
     case GPP_OPT_MAX_BUCKETS:
         if (value < 1 || value > (1 << 20))
             return fail(GPP_ERR_INVALID_ARGUMENT, "max buckets must be in [1, 2^20], got %d", value);
@@ -882,6 +888,7 @@ gpp_status gpp_ctx_get_option(const gpp_ctx *ctx, gpp_option option, int *value)
     case GPP_OPT_REDUCE_FREQS: *value = ctx->red_fc; return GPP_OK;
     case GPP_OPT_MAX_BUCKETS: *value = ctx->max_buckets; return GPP_OK;
     case GPP_OPT_REDUCE_FREQS_PER_BLOCK: *value = ctx->red_fpb; return GPP_OK;
+    case GPP_OPT_FUSED_REDUCE: *value = ctx->fuse_reduce; return GPP_OK;
     default: return fail(GPP_ERR_INVALID_ARGUMENT, "unknown option %d", (int)option);
     }
 }
@@ -920,12 +927,16 @@ gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form)
             HIP_OR_FAIL(hipStreamSynchronize(ctx->stream), "autotune synchronize");
             if (rep == 0) continue;  // warm-up
             float ms = 0.f;
+            double g_ms = 0.0, r_ms = 0.0;
             const gpp::StageTimer::Slot &t = ctx->timer.last();
             for (int i = 0; i < t.n; ++i) {
                 HIP_OR_FAIL(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]), "hipEventElapsedTime");
-                if (strncmp(t.name[i], "factored_gram", 13) == 0) best_g = std::min(best_g, (double)ms);
-                if (strcmp(t.name[i], "factored_pole_reduce") == 0) best_r = std::min(best_r, (double)ms);
+                if (strncmp(t.name[i], "factored_gram", 13) == 0) g_ms = ms;
+                if (strcmp(t.name[i], "factored_pole_reduce") == 0 || strcmp(t.name[i], "factored_pole_cols") == 0)
+                    r_ms = ms;
             }
+            best_g = std::min(best_g, g_ms);
+            best_r = std::min(best_r, r_ms);
         }
         *gram_ms = best_g;
         *red_ms = best_r;
@@ -935,8 +946,10 @@ gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form)
     double g, r, best = 1e300;
     if (form == GPP_FORM_FACTORED_F64)
         for (int tile = 0; tile < gpp::kGramTiles; ++tile) {
+            // Gram plus reduce (with GPP_OPT_FUSED_REDUCE the 128x128 tile
+            // carries the reduce in its epilogue where it applies)
             if ((st = trial(tile, best_chunk, best_fc, &g, &r))) return st;
-            if (g < best) best = g, best_tile = tile;
+            if (g + r < best) best = g + r, best_tile = tile;
         }
     best = 1e300;
     const int chunks[] = {2048, 4096, 8192, 16384};

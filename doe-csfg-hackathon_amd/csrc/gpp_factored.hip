@@ -55,21 +55,39 @@ AchtSplit acht_split(const DevProblem &p)
     return a;
 }
 
+// The fused Gram + pole reduce (k_gram_pole) serves at most three
+// frequencies, none negative, at most kFuseNv occupied bands, without
+// acht_n1_loc, on the 128 x 128 FP64 tile.
+typedef GramCfg<2, 4> FuseCfg;
+
+bool fused_reduce(const DevProblem &p, const GramKind &k)
+{
+    return k.fuse && k.ozaki_slices == 0 && k.tile == 0 && !k.acht && p.nfreq <= 3 && !p.wx_neg &&
+           p.nv <= kFuseNv;
+}
+
 struct FactLayout {
-    size_t bg, m, col_ach, col_asx, part_achs, p2, br1, acht_part, acht_occ, total;
-    int nsplit;
+    size_t bg, m, col_ach, col_asx, col2_ach, col2_asx, part_achs, p2, br1, acht_part, acht_occ, total;
+    int nsplit;  // G-vector ranges of the pole-reduce partials (the fused form: G-vector tiles)
+    bool fused;
     AchtSplit as;
 };
 
 FactLayout fact_layout(const DevProblem &p, const GramKind &k)
 {
     FactLayout L;
-    L.nsplit = (p.ncouls + k.red_chunk - 1) / k.red_chunk;
+    L.fused = fused_reduce(p, k);
+    L.nsplit = L.fused ? (p.ncouls + FuseCfg::TI - 1) / FuseCfg::TI : (p.ncouls + k.red_chunk - 1) / k.red_chunk;
     size_t off = 0;
     L.bg = off;        off += align256(sizeof(cplx) * (size_t)p.nb * p.ngpown);
-    L.m = off;         off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
+    L.m = off;         off += L.fused ? 0 : align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
     L.col_ach = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
     L.col_asx = off;   off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * p.nfreq);
+    L.col2_ach = L.col2_asx = off;
+    if (L.fused) {
+        L.col2_ach = off; off += align256(sizeof(cplx) * (size_t)p.ngpown * p.nfreq);
+        L.col2_asx = off; off += align256(sizeof(cplx) * (size_t)p.ngpown * p.nfreq);
+    }
     L.part_achs = off; off += align256(sizeof(cplx) * ach_region_entries(p.nb, p.ngpown));
     L.p2 = L.br1 = L.acht_part = L.acht_occ = off;
     L.as = AchtSplit{0, 0, 0};
@@ -102,6 +120,37 @@ __global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 
     const int t = xcd_tile(blockIdx.x, nj * ni);
     gram_f64_tile<WJN, WIN>(bg, aqsn, ngpown, ncouls, k_begin, k_end, (t % nj) * C::TJ, (t / nj) * C::TI, sB, sA,
                             mout);
+}
+
+// Stages 2 and 3 fused (at most three frequencies, none negative,
+// acht_n1_loc not requested): the Gram tile of k_gram_f64 in the transposed
+// orientation, then the pole terms of its elements summed per column in the
+// epilogue (gram_pole_epilogue), so Mu never reaches HBM.  Same grid and tile
+// order as k_gram_f64; col_ach/col_asx get one sum per (column, G-vector
+// tile, frequency).
+template <int F>
+__global__ __launch_bounds__(512, 1) void k_gram_pole(PoleTileArgs a)
+{
+    typedef GramCfg<2, 4> C;
+    __shared__ cplx lds[FuseLds<F>::kTotal];
+    auto *sB = reinterpret_cast<cplx(*)[kKS][C::TJ]>(lds);
+    auto *sA = reinterpret_cast<cplx(*)[kKS][C::TI]>(lds + 2 * kKS * C::TJ);
+    const int nj = (a.ngpown + C::TJ - 1) / C::TJ;
+    const int ni = (a.ncouls + C::TI - 1) / C::TI;
+    const int t = xcd_tile(blockIdx.x, nj * ni);
+    const int j0 = (t % nj) * C::TJ, itile = t / nj, i0 = itile * C::TI;
+    v4d acc1[4][2], acc2[4][2], acc3[4][2];
+    gram_f64_acc<2, 4, true>(a.bg, a.aqsn, a.ngpown, a.ncouls, a.nv, a.nb, j0, i0, sB, sA, acc1, acc2, acc3);
+    gram_pole_epilogue<F>(a, j0, i0, itile, acc1, acc2, acc3, lds);
+}
+
+// One wave per column: the fused epilogue's tile sums, summed and scaled.
+__global__ __launch_bounds__(256) void k_pole_cols(const cplx *col_ach, const cplx *col_asx, int nsplit, int nfreq,
+                                                   const int32_t *idx, const double *vcoul, int ncouls, int ngpown,
+                                                   double occ, cplx *out_ach, cplx *out_asx)
+{
+    const int j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (j < ngpown) pole_cols_wave(col_ach, col_asx, nsplit, nfreq, idx, vcoul, ncouls, occ, j, out_ach, out_asx);
 }
 
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
@@ -240,6 +289,14 @@ void launch_reduce_fc(ReduceArgs a, hipStream_t s)
         launch_reduce_p2<FC, false>(a, grid, s);
 }
 
+template <int F>
+void launch_gram_pole_f(const DevProblem &p, const PoleTileArgs &a, hipStream_t s)
+{
+    const unsigned tiles =
+        (unsigned)((p.ngpown + FuseCfg::TJ - 1) / FuseCfg::TJ) * (unsigned)((p.ncouls + FuseCfg::TI - 1) / FuseCfg::TI);
+    hipLaunchKernelGGL((k_gram_pole<F>), dim3(tiles), dim3(FuseCfg::NT), 0, s, a);
+}
+
 #define GPP_TRY(x)                       \
     do {                                 \
         hipError_t e_ = (x);             \
@@ -294,6 +351,30 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     if (timer) timer->begin(s);
     GPP_TRY(launch_achstemp(p, part_achs, bg, s));  // also gathers Bg
     if (timer) timer->mark(s, "factored_gather_achstemp");
+    if (L.fused) {
+        PoleTileArgs pa;
+        pa.aqsn = p.aqsn; pa.ieps = p.ieps; pa.wtilde = p.wtilde; pa.bg = bg; pa.wx = p.wx;
+        pa.nb = p.nb; pa.nv = p.nv; pa.ncouls = p.ncouls; pa.ngpown = p.ngpown; pa.nsplit = L.nsplit;
+        pa.col_ach = col_ach; pa.col_asx = col_asx;
+        switch (p.nfreq) {
+        case 1: launch_gram_pole_f<1>(p, pa, s); break;
+        case 2: launch_gram_pole_f<2>(p, pa, s); break;
+        default: launch_gram_pole_f<3>(p, pa, s); break;
+        }
+        GPP_TRY(hipGetLastError());
+        if (timer) timer->mark(s, "factored_gram_pole");
+        cplx *c2a = reinterpret_cast<cplx *>(w + L.col2_ach);
+        cplx *c2x = reinterpret_cast<cplx *>(w + L.col2_asx);
+        hipLaunchKernelGGL(k_pole_cols, dim3((p.ngpown + 3) / 4), dim3(256), 0, s, col_ach, col_asx, L.nsplit,
+                           p.nfreq, p.idx, p.vcoul, p.ncouls, p.ngpown, p.occ, c2a, c2x);
+        GPP_TRY(hipGetLastError());
+        if (timer) timer->mark(s, "factored_pole_cols");
+        hipLaunchKernelGGL(k_fact_final, dim3(p.nfreq + 1), dim3(kRedThreads), 0, s, c2a, c2x, part_achs, p.ngpown,
+                           p.nfreq, p.nb, p.nv, res.base);
+        GPP_TRY(hipGetLastError());
+        if (timer) timer->mark(s, "factored_final");
+        return hipSuccess;
+    }
     if (kind.ozaki_slices) {
         GPP_TRY(launch_gram_ozaki(p, kind.ozaki_slices, kind.int8, kind.layout, kind.group, bg, w + L.total, mu, s));
         if (timer) timer->mark(s, kind.int8 ? "factored_gram_ozaki_i8" : "factored_gram_ozaki");

@@ -40,11 +40,19 @@ template <int WJN, int WIN> struct GramCfg {
 // three accumulators fed by one MFMA each per k-step instead of two fed by
 // two.  Rounding: |P3| <= 2 sum |b||a|, so the error stays of order
 // 1e-16 sum |b||a| (normwise), as with four products.
-template <int WJN, int WIN>
-__device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
-                                              int ncouls, int k_begin, int k_end, int j0, int i0,
-                                              cplx (*sB)[kKS][GramCfg<WJN, WIN>::TJ],
-                                              cplx (*sA)[kKS][GramCfg<WJN, WIN>::TI], cplx *__restrict__ mout)
+//
+// TRANS swaps the MFMA operands: rows are then G-vectors and columns G'
+// columns, result r of lane l of sub-tile (s, t) is element
+// (ig = wi + 16t + 4r + l/16, j = wj + 16s + l%16) -- each lane holds four
+// G' columns and eight G-vectors of each, the layout the fused pole
+// epilogue sums over G-vectors in.  Every element is the same sum of the
+// same products in either orientation.
+template <int WJN, int WIN, bool TRANS = false>
+__device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
+                                             int ncouls, int k_begin, int k_end, int j0, int i0,
+                                             cplx (*sB)[kKS][GramCfg<WJN, WIN>::TJ],
+                                             cplx (*sA)[kKS][GramCfg<WJN, WIN>::TI], v4d (&acc1)[4][2],
+                                             v4d (&acc2)[4][2], v4d (&acc3)[4][2])
 {
     typedef GramCfg<WJN, WIN> C;
     constexpr int TJ = C::TJ, TI = C::TI, NT = C::NT;
@@ -55,7 +63,6 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
     const int wj = (wave / WIN) * kWJ;
     const int wi = (wave % WIN) * kWI;
 
-    v4d acc1[4][2], acc2[4][2], acc3[4][2];  // P1, P2, P3
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -119,9 +126,15 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
                 const double bdiff = b.re - b.im;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    acc1[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.re, are[t], acc1[s][t], 0, 0, 0);
-                    acc2[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.im, aim[t], acc2[s][t], 0, 0, 0);
-                    acc3[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bdiff, asum[t], acc3[s][t], 0, 0, 0);
+                    if constexpr (TRANS) {
+                        acc1[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[t], b.re, acc1[s][t], 0, 0, 0);
+                        acc2[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[t], b.im, acc2[s][t], 0, 0, 0);
+                        acc3[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(asum[t], bdiff, acc3[s][t], 0, 0, 0);
+                    } else {
+                        acc1[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.re, are[t], acc1[s][t], 0, 0, 0);
+                        acc2[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.im, aim[t], acc2[s][t], 0, 0, 0);
+                        acc3[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bdiff, asum[t], acc3[s][t], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -129,7 +142,21 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
         if (st + 1 < nst) store_stage(buf ^ 1);
         __syncthreads();
     }
+}
 
+// Stage 2 with Mu written to HBM ([ngpown][ncouls], rows contiguous in ig).
+template <int WJN, int WIN>
+__device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
+                                              int ncouls, int k_begin, int k_end, int j0, int i0,
+                                              cplx (*sB)[kKS][GramCfg<WJN, WIN>::TJ],
+                                              cplx (*sA)[kKS][GramCfg<WJN, WIN>::TI], cplx *__restrict__ mout)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int wj = (wave / WIN) * kWJ;
+    const int wi = (wave % WIN) * kWI;
+    v4d acc1[4][2], acc2[4][2], acc3[4][2];  // P1, P2, P3
+    gram_f64_acc<WJN, WIN>(bg, aqsn, ngpown, ncouls, k_begin, k_end, j0, i0, sB, sA, acc1, acc2, acc3);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -142,6 +169,41 @@ __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const
                     mout[(size_t)j * ncouls + ig] =
                         cplx{acc1[s][t][r] + acc2[s][t][r], (acc3[s][t][r] - acc1[s][t][r]) + acc2[s][t][r]};
             }
+}
+
+// 2^-40 band around a threshold inside which a decision made on an estimate
+// is remade in the reference's arithmetic.
+constexpr double kBand = 1.0 / 1099511627776.0;
+
+// The occupied solver's branch tests (gppComplex.cpp:35, :43) on |delw|^2 for
+// one element and frequency: branch1 = |wdiff|^2 > 0.25 and |delw|^2 <
+// limitone, above2 = |delw|^2 > to1.  wdiffr = |wdiff|^2 rounded as the
+// reference rounds it, rden its reciprocal (rcp_1nr), wt2 = |wt|^2.
+// est = wt2 * rden is within 1e-14 of the reference's rounded |delw|^2,
+// relative to it (each rounding of delw is bounded by |delw|, with or
+// without cancellation), so outside a 2^-40 band around the two thresholds
+// it decides as the reference does; inside it, and for a non-finite est, the
+// reference's delw and |delw|^2 are recomputed in its operation order with
+// the correctly rounded reciprocal (:30-32).
+__device__ __forceinline__ void occ_branch_tests(double wxf, cplx wt, double wdiffr, double rden, double wt2,
+                                                 bool &branch1, bool &above2)
+{
+#pragma clang fp contract(off)
+    constexpr double limitone = 1.0 / (kTo1 * 4.0);
+    constexpr double kEstLo1 = limitone * (1 - kBand), kEstHi1 = limitone * (1 + kBand);
+    constexpr double kEstLo2 = kTo1 * (1 - kBand), kEstHi2 = kTo1 * (1 + kBand);
+    const double est = wt2 * rden;
+    const bool sure = (est < kEstLo2) | ((est > kEstHi2) & (est < kEstLo1)) | ((est > kEstHi1) & (est <= __DBL_MAX__));
+    bool below1 = est < kEstLo1;
+    above2 = est > kEstHi2;
+    if (!sure) {
+        const cplx wdiff = cdminus(wxf, wt);
+        const cplx delw = cscale(cmul(wt, cconj(wdiff)), rcp_refined(wdiffr));
+        const double delwr = cmul(delw, cconj(delw)).re;
+        below1 = delwr < limitone;
+        above2 = delwr > kTo1;
+    }
+    branch1 = (wdiffr > 0.25) && below1;
 }
 
 // One instance's inputs and workspace as seen by the reduce stages.
@@ -190,11 +252,6 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
-    const double limitone = 1.0 / (kTo1 * 4.0);
-    // 2^-40 bands around limitone (:22) and to1 (:20) for the |delw|^2 tests
-    constexpr double kBand = 1.0 / 1099511627776.0;
-    constexpr double kEstLo1 = 1.0 / (kTo1 * 4.0) * (1 - kBand), kEstHi1 = 1.0 / (kTo1 * 4.0) * (1 + kBand);
-    constexpr double kEstLo2 = kTo1 * (1 - kBand), kEstHi2 = kTo1 * (1 + kBand);
     // conj(Bg[n1][j]) of the occupied bands, loaded once for the column (the
     // optional p2/br1 stores in the loop would otherwise keep the compiler
     // from hoisting these loads out of it).
@@ -305,29 +362,7 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                     sach[f].im = __builtin_fma(__builtin_fma(u0.im, wdr, u0_re_wti), rden, sach[f].im);
                 }
                 bool branch1 = false, above2 = false;
-                if (nv > 0) {
-                    // The branch tests of :35 and :43 on |delw|^2.  est = |wt|^2 /
-                    // |wdiff|^2 is within 1e-14 of the reference's rounded |delw|^2,
-                    // relative to it (each rounding of delw is bounded by |delw|,
-                    // with or without cancellation), so outside a 2^-40 band
-                    // around the two thresholds it decides as the reference does;
-                    // inside it, and for a non-finite est, the reference's delw
-                    // and |delw|^2 are recomputed in its operation order with the
-                    // correctly rounded reciprocal (:30-32).
-                    const double est = wt2 * rden;
-                    const bool sure = (est < kEstLo2) | ((est > kEstHi2) & (est < kEstLo1)) |
-                                      ((est > kEstHi1) & (est <= __DBL_MAX__));
-                    bool below1 = est < kEstLo1;
-                    above2 = est > kEstHi2;
-                    if (!sure) {
-                        const cplx wdiff = cdminus(wxv[f], wt);
-                        const cplx delw = cscale(cmul(wt, cconj(wdiff)), rcp_refined(wdiffr));
-                        const double delwr = cmul(delw, cconj(delw)).re;
-                        below1 = delwr < limitone;
-                        above2 = delwr > kTo1;
-                    }
-                    branch1 = (wdiffr > 0.25) && below1;
-                }
+                if (nv > 0) occ_branch_tests(wxv[f], wt, wdiffr, rden, wt2, branch1, above2);
                 // Branch 2 (:46-50) with cden = 4 wt^2 (0.5 + delw), Omega2 =
                 // wt^2 I_eps and delw = wt / (wx - wt):
                 //   -Omega2 conj(cden) delw / |cden|^2 = -I_eps delw / (2 + 4 delw)
@@ -422,6 +457,241 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
     } else {
         for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), fold_on(), iw0);
         if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_on(), iw0);
+    }
+}
+
+// One element's pole terms at F frequencies, evaluated as pole_reduce_item
+// does with fewer than four frequencies and none negative: P = delw I_eps
+// formed per frequency, sch = P Mu, or P (Mu + Mo) in branch 1; ssx from the
+// branch (branch 1: c1 Omega2; branch 2: the reduced form -(q / 2) /
+// (wx + wt)) times Mo.  occ: the problem has occupied bands.
+template <int F>
+__device__ __forceinline__ void pole_element_plain(const double (&wxv)[F], const double (&c1)[F], bool occ, cplx wt,
+                                                   cplx ie, cplx mu, cplx mo, cplx (&sach)[F], cplx (&sasx)[F])
+{
+#pragma clang fp contract(off)
+    const cplx q = cmul_fast(wt, ie);
+    const double wti2 = wt.im * wt.im;  // rounded as the reference's |wdiff|^2 rounds it
+    const double q_im_wti = q.im * wt.im, q_re_wti = q.re * wt.im;
+    cplx om2 = czero(), mum = mu;
+    double wt2 = 0.0;
+    if (occ) {
+        om2 = cmul(csquare(wt), ie);  // Omega2 = wtilde^2 * I_eps, flagOCC_solver :117-118
+        mum = cadd(mu, mo);
+        wt2 = __builtin_fma(wt.re, wt.re, wti2);
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        const double wdr = wxv[f] - wt.re;       // wdiff = (wdr, -wt.im), :27
+        const double wdiffr = wdr * wdr + wti2;  // |wdiff|^2 exactly as the reference rounds it
+        const double rden = rcp_1nr(wdiffr);
+        bool branch1 = false, above2 = false;
+        if (occ) occ_branch_tests(wxv[f], wt, wdiffr, rden, wt2, branch1, above2);
+        const cplx p = {__builtin_fma(q.re, wdr, -q_im_wti) * rden, __builtin_fma(q.im, wdr, q_re_wti) * rden};
+        if (occ) {
+            cplx ssx = czero();
+            if (branch1) {
+                ssx = cscale(om2, c1[f]);
+            } else if (above2) {
+                const double sr = wxv[f] + wt.re;
+                const double rr = 0.5 * rcp_1nr(__builtin_fma(sr, sr, wti2));
+                ssx = cplx{-__builtin_fma(q.re, sr, q_im_wti) * rr, -__builtin_fma(q.im, sr, -q_re_wti) * rr};
+            }
+            sasx[f] = cfma(ssx, mo, sasx[f]);
+        }
+        sach[f] = cfma(p, branch1 ? mum : mu, sach[f]);
+    }
+}
+
+// Inputs and outputs of the Gram product with the pole reduce fused into its
+// epilogue (at most 3 frequencies, none negative, acht_n1_loc not requested).
+struct PoleTileArgs {
+    const cplx *aqsn, *ieps, *wtilde;
+    const cplx *bg;  // [nb][ngpown]
+    const double *wx;
+    int nb, nv, ncouls, ngpown, nsplit;  // nsplit = G-vector tiles
+    cplx *col_ach, *col_asx;             // [ngpown][nsplit][F], unscaled tile sums
+};
+
+// LDS of the fused Gram + pole reduce block (128 x 128 tile, 8 waves), in
+// complex entries: the Gram's double-buffered band stages (sB, sA: 4096
+// entries) during the product; afterwards, for the epilogue rounds, wtilde,
+// I_eps and Mu of 32 columns x 64 G-vectors (rows padded to 65 entries, so
+// the transposed reads of 8 lanes fall in 8 different 16-byte bank groups),
+// conj(Bg) and aqsn of the tile's occupied bands (at most kFuseNv), and the
+// per-wave column sums of one column round.
+constexpr int kFuseNv = 4;
+template <int F> struct FuseLds {
+    typedef GramCfg<2, 4> C;
+    static constexpr int kCols = 32, kRows = 64;   // columns x G-vectors per round
+    static constexpr int kPitch = kRows + 1;       // entries per staged row
+    static constexpr int kWt = 0, kIe = kCols * kPitch, kMu = 2 * kCols * kPitch;
+    static constexpr int kBg = 3 * kCols * kPitch;     // [kFuseNv][TJ]
+    static constexpr int kAq = kBg + kFuseNv * C::TJ;  // [kFuseNv][TI]
+    static constexpr int kRed = kAq + kFuseNv * C::TI;  // [WIN = 4][kCols][2F]
+    static constexpr int kTotal = kRed + 4 * kCols * 2 * F;
+    static_assert(2 * kKS * (C::TJ + C::TI) <= kBg, "the Gram stages fit below the occupied-band area");
+};
+
+// Epilogue of one 128 x 128 tile (G-vector tile `itile`) after
+// gram_f64_acc<TRANS>: the accumulators are folded into Mu at once (128
+// registers instead of 192), and the elements are evaluated in eight rounds
+// (s, t): 32 columns (sub-tile s of both 64-column halves) x 64 G-vectors
+// (sub-tile t of the four 32-G-vector quarters).  In a round all 512 threads
+// stage its wtilde and I_eps rows into LDS with coalesced loads (the tile's
+// occupied conj(Bg) and aqsn with the first), each wave writes its Mu of the
+// round beside them, and each lane evaluates its four elements of the round
+// in a loop -- Mo over the occupied bands (four FMAs per band), the pole
+// terms at the F frequencies (pole_element_plain) -- summing per column over
+// both rounds of a column sub-tile; then the four lane groups are summed by
+// xor butterfly (every lane of a column ends with the same bits) and the
+// four waves of a column range in wave order: one sum per (column, tile,
+// frequency), written unscaled.
+template <int F>
+__device__ __forceinline__ void gram_pole_epilogue(const PoleTileArgs &a, int j0, int i0, int itile,
+                                                   const v4d (&acc1)[4][2], const v4d (&acc2)[4][2],
+                                                   const v4d (&acc3)[4][2], cplx *lds)
+{
+#pragma clang fp contract(off)
+    typedef FuseLds<F> L;
+    typedef GramCfg<2, 4> C;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int wjn = wave / 4, wv = wave % 4;  // 64-column half, 32-G-vector quarter
+    const int ncouls = a.ncouls, ngpown = a.ngpown, nv = a.nv;
+
+    cplx mu[4][2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                mu[s][t][r] = cplx{acc1[s][t][r] + acc2[s][t][r], (acc3[s][t][r] - acc1[s][t][r]) + acc2[s][t][r]};
+
+    double wxv[F], c1[F];  // branch-1 ssx = Omega2 * wx^2 / wx^4 (:38-41) = Omega2 * c1
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        wxv[f] = a.wx[f];
+        const double wx2 = wxv[f] * wxv[f];
+        c1[f] = wx2 * (1.00 / (wx2 * wx2));
+    }
+    // tile-local column of staged row c and G-vector of staged entry q in round (s, t)
+    auto col_of = [](int c, int s) { return (c >> 4) * kWJ + 16 * s + (c & 15); };
+    auto gv_of = [](int q, int t) { return (q >> 4) * kWI + 16 * t + (q & 15); };
+    constexpr int PER = L::kCols * L::kRows / C::NT;  // staged entries per thread and array (4)
+    const int jc = wjn * 16 + lane % 16;              // this lane's staged row
+
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int jl = col_of(jc, s);
+        cplx sach[F], sasx[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) sach[f] = sasx[f] = czero();
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+#pragma unroll
+            for (int arr = 0; arr < 2; ++arr) {
+                const cplx *src = arr ? a.ieps : a.wtilde;
+                cplx v[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int e = threadIdx.x + k * C::NT;
+                    const int j = j0 + col_of(e / L::kRows, s), ig = i0 + gv_of(e % L::kRows, t);
+                    v[k] = (j < ngpown && ig < ncouls) ? src[(size_t)j * ncouls + ig] : czero();
+                }
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int e = threadIdx.x + k * C::NT;
+                    lds[(arr ? L::kIe : L::kWt) + (e / L::kRows) * L::kPitch + e % L::kRows] = v[k];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) lds[L::kMu + jc * L::kPitch + wv * 16 + 4 * r + lane / 16] = mu[s][t][r];
+            if (s == 0 && t == 0) {
+                for (int e = threadIdx.x; e < nv * C::TJ; e += C::NT) {
+                    const int n = e / C::TJ, c = e % C::TJ;
+                    lds[L::kBg + e] = j0 + c < ngpown ? cconj(a.bg[(size_t)n * ngpown + j0 + c]) : czero();
+                    lds[L::kAq + e] = i0 + c < ncouls ? a.aqsn[(size_t)n * ncouls + i0 + c] : czero();
+                }
+            }
+            __syncthreads();
+            if (j0 + jl < ngpown) {
+#pragma unroll 1
+                for (int r = 0; r < 4; ++r) {
+                    const int q = wv * 16 + 4 * r + lane / 16;  // staged entry
+                    const int igl = gv_of(q, t);
+                    if (i0 + igl < ncouls) {
+                        cplx mo = czero();  // occupied bands in band order
+                        for (int n1 = 0; n1 < nv; ++n1)
+                            mo = cfma(lds[L::kBg + n1 * C::TJ + jl], lds[L::kAq + n1 * C::TI + igl], mo);
+                        const int o = jc * L::kPitch + q;
+                        pole_element_plain<F>(wxv, c1, nv > 0, lds[L::kWt + o], lds[L::kIe + o], lds[L::kMu + o], mo,
+                                              sach, sasx);
+                    }
+                }
+            }
+            // every wave's reads of the round's rows are done before the next
+            // round stages over them
+            __syncthreads();
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+#pragma unroll
+            for (int m = 16; m < kWave; m <<= 1) {
+                sach[f].re += __shfl_xor(sach[f].re, m, kWave);
+                sach[f].im += __shfl_xor(sach[f].im, m, kWave);
+                sasx[f].re += __shfl_xor(sasx[f].re, m, kWave);
+                sasx[f].im += __shfl_xor(sasx[f].im, m, kWave);
+            }
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                lds[L::kRed + (wv * L::kCols + jc) * 2 * F + f] = sach[f];
+                lds[L::kRed + (wv * L::kCols + jc) * 2 * F + F + f] = sasx[f];
+            }
+        }
+        // the next column round writes its sums only after two more barriers
+        __syncthreads();
+        for (int e = threadIdx.x; e < L::kCols * 2 * F; e += C::NT) {
+            const int c = e / (2 * F), k = e % (2 * F);
+            const int j = j0 + col_of(c, s);
+            if (j >= ngpown) continue;
+            cplx v = lds[L::kRed + c * 2 * F + k];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) v = cadd(v, lds[L::kRed + (w * L::kCols + c) * 2 * F + k]);
+            const size_t o = ((size_t)j * a.nsplit + itile) * F + (k % F);
+            if (k < F)
+                a.col_ach[o] = v;
+            else
+                a.col_asx[o] = v;
+        }
+    }
+}
+
+// Column sums of the fused epilogue's tile sums: one wave per column j, the
+// tiles summed in a fixed order (strided per lane, then xor butterfly), then
+// scaled as the reduce scales them, (0.5 * sum) * vcoul and sum * (occ *
+// vcoul) (gppComplex.cpp:361-363, :379); out [ngpown][nfreq].
+__device__ __forceinline__ void pole_cols_wave(const cplx *col_ach, const cplx *col_asx, int nsplit, int nfreq,
+                                               const int32_t *idx, const double *vcoul, int ncouls, double occ,
+                                               int j, cplx *out_ach, cplx *out_asx)
+{
+    const int lane = threadIdx.x % kWave;
+    const double vc = vcoul[igp_of(idx, j, ncouls)];
+    for (int f = 0; f < nfreq; ++f) {
+        cplx s = czero(), x = czero();
+        for (int it = lane; it < nsplit; it += kWave) {
+            s = cadd(s, col_ach[((size_t)j * nsplit + it) * nfreq + f]);
+            x = cadd(x, col_asx[((size_t)j * nsplit + it) * nfreq + f]);
+        }
+        s = wave_sum(s);
+        x = wave_sum(x);
+        if (lane == 0) {
+            out_ach[(size_t)j * nfreq + f] = cscale(cscale(s, 0.5), vc);
+            out_asx[(size_t)j * nfreq + f] = cscale(x, occ * vc);
+        }
     }
 }
 

@@ -130,6 +130,7 @@ struct GramKind {
     int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
     int red_fpb = 0;       // frequencies per pole-reduce block (0: all, passes looped in the block)
     bool acht = false;     // also produce acht_n1_loc (a second product, R * aqsn^T)
+    bool fuse = false;     // pole reduce in the FP64 Gram epilogue where it applies (no Mu workspace)
 };
 constexpr int kGramTiles = 4;
 

@@ -41,6 +41,7 @@ GPP_OPT_REDUCE_FREQS = 5
 GPP_OPT_MAX_BUCKETS = 6
 GPP_OPT_REDUCE_FREQS_PER_BLOCK = 7
 GPP_OPT_OZAKI_GROUP = 8
+GPP_OPT_FUSED_REDUCE = 9
 
 # gpp_form
 FORMS = {

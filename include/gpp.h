@@ -133,7 +133,16 @@ typedef enum {
     /* G' column tiles per tile group of the Ozaki Gram kernel: the blocks of
      * a group share those tiles' digit planes in L2 and stream the G-vector
      * digit planes once per group.  0 = sized for the L2 (default). */
-    GPP_OPT_OZAKI_GROUP = 8
+    GPP_OPT_OZAKI_GROUP = 8,
+    /* GPP_FORM_FACTORED_F64 with at most 3 frequencies, none negative, at
+     * most 4 occupied bands, the 128x128 Gram tile and acht_n1_loc not
+     * requested: 1 = evaluate the pole factors in the epilogue of the Gram
+     * product, so the G'xG product Mu (ngpown * ncouls * 16 bytes of
+     * workspace) is never stored -- for problems whose Mu does not fit in
+     * device memory; slower where it fits (512/2/32768/1638: 3.58 against
+     * 3.19 ms); 0 = store Mu and run the pole reduce as its own stage
+     * (default). */
+    GPP_OPT_FUSED_REDUCE = 9
 } gpp_option;
 
 /* Outputs of one sigma evaluation (caller-owned host memory, overwritten).

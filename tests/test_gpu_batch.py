@@ -10,7 +10,7 @@ import pytest
 
 import gpp_inputs
 import hiprt
-from test_gpu_parity import assert_close, problem_of
+from test_gpu_parity import assert_close, problem_of, separate_reduce
 
 pytestmark = pytest.mark.gpu
 
@@ -72,18 +72,23 @@ def test_batch_vs_oracle(ctx, oracle):
 
 
 def test_batch_bitwise_equals_single_path(ctx):
+    """The batch runs the Gram product and the pole reduce as separate
+    stages, as the single path does by default (GPP_OPT_FUSED_REDUCE = 0):
+    bitwise equal to it (the fused single path is compared in
+    test_gpu_fused)."""
     import pygpp
     probs = [pygpp.Problem(*s) for s in SHAPES]
     seeds = [11 + i for i in range(len(probs))]
     with pygpp.Batch(ctx, probs, seeds=seeds) as b:
         outs = b.sigma()
         cfg = b.last_timing()
-    for p, s, out in zip(probs, seeds, outs):
-        ctx.generate_seeded(p, s)
-        ref = ctx.sigma("factored-f64")
-        assert cfg["gram_tile"] == ctx.get_option(pygpp.GPP_OPT_GRAM_TILE)
-        for k in KEYS:
-            np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    with separate_reduce(ctx):
+        for p, s, out in zip(probs, seeds, outs):
+            ctx.generate_seeded(p, s)
+            ref = ctx.sigma("factored-f64")
+            assert cfg["gram_tile"] == ctx.get_option(pygpp.GPP_OPT_GRAM_TILE)
+            for k in KEYS:
+                np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
 
 
 @pytest.mark.parametrize("tile", [0, 3])
@@ -99,11 +104,12 @@ def test_batch_gram_tiles_bitwise(ctx, tile):
         with pygpp.Batch(ctx, probs, seeds=seeds) as b:
             outs = b.sigma()
             assert b.last_timing()["gram_tile"] == tile
-        for p, s, out in zip(probs, seeds, outs):
-            ctx.generate_seeded(p, s)
-            ref = ctx.sigma("factored-f64")
-            for k in KEYS:
-                np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+        with separate_reduce(ctx):
+            for p, s, out in zip(probs, seeds, outs):
+                ctx.generate_seeded(p, s)
+                ref = ctx.sigma("factored-f64")
+                for k in KEYS:
+                    np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
     finally:
         ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
 
@@ -230,6 +236,7 @@ def test_batch_negative_frequency_instances(ctx, oracle, form):
         else:
             assert_close_modulus(out, ref, TOL, KEYS)
         ctx.upload(problem_of(a, nv), a)
-        single = ctx.sigma(form)
+        with separate_reduce(ctx):
+            single = ctx.sigma(form)
         for k in KEYS:
             np.testing.assert_array_equal(out[k], single[k], err_msg=k)

@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import gpp_inputs
-from test_gpu_parity import assert_close, assert_close_conditioned, comp_rel_err, problem_of, run_gpu
+from test_gpu_parity import (assert_close, assert_close_conditioned, comp_rel_err, fused_reduce, problem_of, run_gpu,
+                             separate_reduce)
 
 pytestmark = pytest.mark.gpu
 
@@ -119,14 +120,16 @@ ACHT_SHAPES = SHAPES + [
 def test_factored_acht_n1_loc_vs_oracle(ctx, oracle, shape, form):
     """acht_n1_loc from the second product (pole factor of frequency 2, branch
     mask for the occupied bands) against the oracle's per-band sums; asking for
-    it leaves the other outputs bit-identical."""
+    it leaves the other outputs bit-identical to those of the separate pole
+    reduce (the stages it runs)."""
     nb, nv, ncouls, ngpown, nfreq = shape
     arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=sum(shape) + 11,
                                     dw=0.25 if nfreq > 5 else 1.0)
     ctx.upload(problem_of(arrs, nv), arrs)
     out = ctx.sigma(form, acht_n1_loc=True)
     assert_close(out, oracle.sigma(arrs, nvband=nv, threads=8), TOL_FACTORED, KEYS + ("acht_n1_loc",))
-    plain = ctx.sigma(form)
+    with separate_reduce(ctx):
+        plain = ctx.sigma(form)
     for k in KEYS:
         np.testing.assert_array_equal(out[k], plain[k])
 
@@ -295,20 +298,24 @@ GRAM_TILE_SHAPES = [
 @pytest.mark.parametrize("shape", GRAM_TILE_SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_gram_tiles_agree(ctx, oracle, shape):
     """The four FP64 Gram tilings accumulate every Mu entry over the bands in
-    the same order: bit-identical outputs (with and without acht_n1_loc),
-    within 1e-10 of the oracle."""
+    the same order: bit-identical outputs (with and without acht_n1_loc; the
+    pole reduce as its own stage), within 1e-10 of the oracle, as is the
+    128 x 128 tile with the reduce in its epilogue."""
     import pygpp
     nb, nv, ncouls, ngpown, nfreq = shape
     arrs = gpp_inputs.seeded_inputs(nb, ncouls, ngpown, nfreq=nfreq, seed=nb + ngpown)
     ctx.upload(problem_of(arrs, nv), arrs)
     out = {}
     try:
-        for tile in range(4):
-            ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, tile)
-            out[tile] = ctx.sigma("factored-f64")
-            out[(tile, "acht")] = ctx.sigma("factored-f64", acht_n1_loc=True)
+        with separate_reduce(ctx):
+            for tile in range(4):
+                ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, tile)
+                out[tile] = ctx.sigma("factored-f64")
+                out[(tile, "acht")] = ctx.sigma("factored-f64", acht_n1_loc=True)
     finally:
         ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
+    with fused_reduce(ctx, True):
+        assert_close(ctx.sigma("factored-f64"), oracle.sigma(arrs, nvband=nv, threads=8), TOL_FACTORED, KEYS)
     for tile in range(1, 4):
         for k in KEYS:
             np.testing.assert_array_equal(out[tile][k], out[0][k], err_msg=f"tile {tile} {k}")

@@ -3,6 +3,8 @@ oracle and with the reference's own outputs.  Tolerance: 1e-12 relative per
 component (the direct kernel's bound against the reference CPU build); a
 component smaller than 1e-6 of its complex modulus is compared relative to
 the modulus."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -54,6 +56,26 @@ def assert_close_conditioned(out, seq, dd, tol=TOL, keys=("achtemp", "asxtemp"))
             assert np.all(np.abs(gc - dc) <= tol * scale + own), \
                 f"{k}: vs compensated oracle {np.abs(gc - dc) / scale} > {tol} + {own / scale}"
             assert np.all(np.abs(gc - sc) <= tol * scale + 2 * own), f"{k}: vs sequential oracle"
+
+
+@contextlib.contextmanager
+def fused_reduce(ctx, on):
+    """GPP_OPT_FUSED_REDUCE set to `on` for the block, restored after."""
+    import pygpp
+    before = ctx.get_option(pygpp.GPP_OPT_FUSED_REDUCE)
+    ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, 1 if on else 0)
+    try:
+        yield
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, before)
+
+
+def separate_reduce(ctx):
+    """The factored FP64 form with the Gram product stored and the pole
+    reduce as its own stage (the default) -- the stages the batched form and
+    the other Gram tiles run -- for bitwise comparisons with them, whatever
+    the context's setting."""
+    return fused_reduce(ctx, False)
 
 
 def problem_of(arrs, nv):
