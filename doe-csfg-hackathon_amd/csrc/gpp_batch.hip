@@ -68,8 +68,8 @@ __global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 
                                                                const GramItem *__restrict__ items, int nitems)
 {
     typedef GramCfg<WJN, WIN> C;
-    __shared__ cplx sB[2][kKS][C::TJ];
-    __shared__ cplx sA[2][kKS][C::TI];
+    __shared__ cplx sB[2][C::KS][C::TJ];
+    __shared__ cplx sA[2][C::KS][C::TI];
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const GramItem it = items[w];
         const BInst &b = insts[it.inst];

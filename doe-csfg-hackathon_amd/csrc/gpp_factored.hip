@@ -113,8 +113,8 @@ __global__ __launch_bounds__(64 * WJN * WIN, WJN * WIN >= 4 ? 8 / (WJN * WIN) : 
                                                                     cplx *__restrict__ mout)
 {
     typedef GramCfg<WJN, WIN> C;
-    __shared__ cplx sB[2][kKS][C::TJ];
-    __shared__ cplx sA[2][kKS][C::TI];
+    __shared__ cplx sB[2][C::KS][C::TJ];
+    __shared__ cplx sA[2][C::KS][C::TI];
     const int nj = (ngpown + C::TJ - 1) / C::TJ;
     const int ni = (ncouls + C::TI - 1) / C::TI;
     const int t = xcd_tile(blockIdx.x, nj * ni);
@@ -133,8 +133,8 @@ __global__ __launch_bounds__(512, 1) void k_gram_pole(PoleTileArgs a)
 {
     typedef GramCfg<2, 4> C;
     __shared__ cplx lds[FuseLds<F>::kTotal];
-    auto *sB = reinterpret_cast<cplx(*)[kKS][C::TJ]>(lds);
-    auto *sA = reinterpret_cast<cplx(*)[kKS][C::TI]>(lds + 2 * kKS * C::TJ);
+    auto *sB = reinterpret_cast<cplx(*)[C::KS][C::TJ]>(lds);
+    auto *sA = reinterpret_cast<cplx(*)[C::KS][C::TI]>(lds + 2 * C::KS * C::TJ);
     const int nj = (a.ngpown + C::TJ - 1) / C::TJ;
     const int ni = (a.ncouls + C::TI - 1) / C::TI;
     const int t = xcd_tile(blockIdx.x, nj * ni);

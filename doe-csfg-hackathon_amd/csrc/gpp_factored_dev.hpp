@@ -20,13 +20,17 @@ namespace fact {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int kKS = 8;             // bands per LDS stage of the FP64 Gram tile (two MFMA k-steps)
+constexpr int kKS = 8;             // bands per LDS stage of the smaller FP64 Gram tiles (two MFMA k-steps)
 constexpr int kWJ = 64, kWI = 32;  // FP64 Gram wave tile: 64 G' columns x 32 G-vectors
 constexpr int kRedThreads = 256;   // pole-reduce and final-reduce blocks
 
-// FP64 Gram block tile made of WJN x WIN waves.
+// FP64 Gram block tile made of WJN x WIN waves; KS bands per LDS stage.  The
+// 128 x 128 tile stages 12 bands (three k-steps per barrier; its staging
+// registers still fit two waves per SIMD), the smaller tiles 8, where 12
+// would spill.
 template <int WJN, int WIN> struct GramCfg {
     static constexpr int TJ = kWJ * WJN, TI = kWI * WIN, NT = 64 * WJN * WIN;
+    static constexpr int KS = WJN * WIN == 8 ? 12 : kKS;
 };
 
 // Stage 2: one TJ x TI tile of Mu = conj(Bg)^T * aqsn over bands [k_begin, k_end)
@@ -50,14 +54,14 @@ template <int WJN, int WIN> struct GramCfg {
 template <int WJN, int WIN, bool TRANS = false>
 __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
                                              int ncouls, int k_begin, int k_end, int j0, int i0,
-                                             cplx (*sB)[kKS][GramCfg<WJN, WIN>::TJ],
-                                             cplx (*sA)[kKS][GramCfg<WJN, WIN>::TI], v4d (&acc1)[4][2],
+                                             cplx (*sB)[GramCfg<WJN, WIN>::KS][GramCfg<WJN, WIN>::TJ],
+                                             cplx (*sA)[GramCfg<WJN, WIN>::KS][GramCfg<WJN, WIN>::TI], v4d (&acc1)[4][2],
                                              v4d (&acc2)[4][2], v4d (&acc3)[4][2])
 {
     typedef GramCfg<WJN, WIN> C;
-    constexpr int TJ = C::TJ, TI = C::TI, NT = C::NT;
-    constexpr int PB = kKS * TJ / NT, PA = kKS * TI / NT;  // staged elements per thread
-    static_assert(PB * NT == kKS * TJ && PA * NT == kKS * TI, "stage must split evenly");
+    constexpr int TJ = C::TJ, TI = C::TI, NT = C::NT, KS = C::KS;
+    constexpr int PB = KS * TJ / NT, PA = KS * TI / NT;  // staged elements per thread
+    static_assert(PB * NT == KS * TJ && PA * NT == KS * TI, "stage must split evenly");
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wj = (wave / WIN) * kWJ;
@@ -96,19 +100,19 @@ __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const 
         }
     };
 
-    const int nst = (k_end - k_begin + kKS - 1) / kKS;
+    const int nst = (k_end - k_begin + KS - 1) / KS;
     load_stage(k_begin);
     store_stage(0);
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nst) load_stage(k_begin + (st + 1) * kKS);
+        if (st + 1 < nst) load_stage(k_begin + (st + 1) * KS);
         __builtin_amdgcn_s_setprio(1);  // the MFMA section: the wave with ready operands issues first
         // scheduling hint: interleave the fragment reads from LDS with the MFMAs
         // (strategy 1; measured ~1 % faster than the default schedule and than strategy 0)
         __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
-        for (int kk = 0; kk < kKS; kk += 4) {
+        for (int kk = 0; kk < KS; kk += 4) {
             const int kr = kk + lane / 16;
             double are[2], aim[2], asum[2];
 #pragma unroll
@@ -148,8 +152,9 @@ __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const 
 template <int WJN, int WIN>
 __device__ __forceinline__ void gram_f64_tile(const cplx *__restrict__ bg, const cplx *__restrict__ aqsn, int ngpown,
                                               int ncouls, int k_begin, int k_end, int j0, int i0,
-                                              cplx (*sB)[kKS][GramCfg<WJN, WIN>::TJ],
-                                              cplx (*sA)[kKS][GramCfg<WJN, WIN>::TI], cplx *__restrict__ mout)
+                                              cplx (*sB)[GramCfg<WJN, WIN>::KS][GramCfg<WJN, WIN>::TJ],
+                                              cplx (*sA)[GramCfg<WJN, WIN>::KS][GramCfg<WJN, WIN>::TI],
+                                              cplx *__restrict__ mout)
 {
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
@@ -530,7 +535,7 @@ template <int F> struct FuseLds {
     static constexpr int kAq = kBg + kFuseNv * C::TJ;  // [kFuseNv][TI]
     static constexpr int kRed = kAq + kFuseNv * C::TI;  // [WIN = 4][kCols][2F]
     static constexpr int kTotal = kRed + 4 * kCols * 2 * F;
-    static_assert(2 * kKS * (C::TJ + C::TI) <= kBg, "the Gram stages fit below the occupied-band area");
+    static_assert(2 * C::KS * (C::TJ + C::TI) <= kBg, "the Gram stages fit below the occupied-band area");
 };
 
 // Epilogue of one 128 x 128 tile (G-vector tile `itile`) after
