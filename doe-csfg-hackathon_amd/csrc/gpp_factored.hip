@@ -154,12 +154,16 @@ __global__ __launch_bounds__(256) void k_pole_cols(const cplx *col_ach, const cp
 }
 
 // Grid (ngpown, nsplit, ceil(nfreq/FC)).  NV: occupied band count fixed at
-// compile time (0: any).  Three waves per SIMD for every FC: with the
-// wave-uniform pole inputs held in scalar registers the FC = 8 variants fit
-// 168 VGPRs (the 8-occupied-band one spills 10), and at 16 frequencies x 8
-// occupied bands three waves run the reduce 12% faster than two.
+// compile time (0: any).  Three waves per SIMD: with the wave-uniform pole
+// inputs held in scalar registers the FC = 8 variants fit 168 VGPRs (the
+// 8-occupied-band one spills 10), and at 16 frequencies x 8 occupied bands
+// three waves run the reduce 12% faster than two.  Four for the plain
+// kernels of at most three frequencies: their waves wait on memory half of
+// the time, and a fourth hides more of it despite 17 spilled registers
+// (3 frequencies x 2 occupied bands: 0.510 -> 0.486 ms at 512/2/32768/1638);
+// the p2-storing and negative-frequency variants spill 30-44 and keep three.
 template <int FC, int NV, bool P2, bool NEG = false>
-__global__ __launch_bounds__(kRedThreads, 3) void k_pole_reduce(ReduceArgs a)
+__global__ __launch_bounds__(kRedThreads, (FC <= 3 && !P2 && !NEG) ? 4 : 3) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
