@@ -132,7 +132,7 @@ template <int F>
 __global__ __launch_bounds__(512, 1) void k_gram_pole(PoleTileArgs a)
 {
     typedef GramCfg<2, 4> C;
-    __shared__ cplx lds[FuseLds<F>::kTotal];
+    __shared__ cplx lds[FuseLds<F>::kLds];
     auto *sB = reinterpret_cast<cplx(*)[C::KS][C::TJ]>(lds);
     auto *sA = reinterpret_cast<cplx(*)[C::KS][C::TI]>(lds + 2 * C::KS * C::TJ);
     const int nj = (a.ngpown + C::TJ - 1) / C::TJ;

@@ -24,13 +24,15 @@ constexpr int kKS = 8;             // bands per LDS stage of the smaller FP64 Gr
 constexpr int kWJ = 64, kWI = 32;  // FP64 Gram wave tile: 64 G' columns x 32 G-vectors
 constexpr int kRedThreads = 256;   // pole-reduce and final-reduce blocks
 
-// FP64 Gram block tile made of WJN x WIN waves; KS bands per LDS stage.  The
-// 128 x 128 tile stages 12 bands (three k-steps per barrier; its staging
-// registers still fit two waves per SIMD), the smaller tiles 8, where 12
-// would spill.
+// FP64 Gram block tile made of WJN x WIN waves; KS bands per LDS stage,
+// staged through registers in NH pieces.  The 128 x 128 tile stages 16 bands
+// (four k-steps per barrier) in two pieces of 8, each stored to the other
+// buffer after its k-steps, so its staging registers fit two waves per SIMD;
+// the smaller tiles stage 8 in one piece (more would spill).
 template <int WJN, int WIN> struct GramCfg {
     static constexpr int TJ = kWJ * WJN, TI = kWI * WIN, NT = 64 * WJN * WIN;
-    static constexpr int KS = WJN * WIN == 8 ? 12 : kKS;
+    static constexpr int KS = WJN * WIN == 8 ? 16 : kKS;
+    static constexpr int NH = WJN * WIN == 8 ? 2 : 1;
 };
 
 // Stage 2: one TJ x TI tile of Mu = conj(Bg)^T * aqsn over bands [k_begin, k_end)
@@ -59,9 +61,11 @@ __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const 
                                              v4d (&acc2)[4][2], v4d (&acc3)[4][2])
 {
     typedef GramCfg<WJN, WIN> C;
-    constexpr int TJ = C::TJ, TI = C::TI, NT = C::NT, KS = C::KS;
-    constexpr int PB = KS * TJ / NT, PA = KS * TI / NT;  // staged elements per thread
-    static_assert(PB * NT == KS * TJ && PA * NT == KS * TI, "stage must split evenly");
+    constexpr int TJ = C::TJ, TI = C::TI, NT = C::NT, KS = C::KS, NH = C::NH;
+    constexpr int PB = KS * TJ / NT / NH, PA = KS * TI / NT / NH;  // staged elements per thread and piece
+    static_assert(PB * NT * NH == KS * TJ && PA * NT * NH == KS * TI, "stage must split evenly");
+    static_assert((PB * NT) % TJ == 0 && (PA * NT) % TI == 0 && (KS / NH) % 4 == 0,
+                  "a piece is whole band rows and whole k-steps");
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wj = (wave / WIN) * kWJ;
@@ -72,47 +76,53 @@ __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const 
 #pragma unroll
         for (int t = 0; t < 2; ++t) acc1[s][t] = acc2[s][t] = acc3[s][t] = v4d{0, 0, 0, 0};
 
+    // piece h of a stage: band rows [h KS / NH, (h + 1) KS / NH)
     cplx stB[PB], stA[PA];
-    auto load_stage = [&](int k0) {
+    auto load_piece = [&](int k0, int h) {
 #pragma unroll
         for (int q = 0; q < PB; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + (h * PB + q) * NT;
             const int kr = e / TJ, c = e % TJ, k = k0 + kr, j = j0 + c;
             stB[q] = (k < k_end && j < ngpown) ? bg[(size_t)k * ngpown + j] : czero();
         }
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + (h * PA + q) * NT;
             const int kr = e / TI, c = e % TI, k = k0 + kr, ig = i0 + c;
             stA[q] = (k < k_end && ig < ncouls) ? aqsn[(size_t)k * ncouls + ig] : czero();
         }
     };
-    auto store_stage = [&](int buf) {
+    auto store_piece = [&](int buf, int h) {
 #pragma unroll
         for (int q = 0; q < PB; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + (h * PB + q) * NT;
             sB[buf][e / TJ][e % TJ] = stB[q];
         }
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
-            const int e = threadIdx.x + q * NT;
+            const int e = threadIdx.x + (h * PA + q) * NT;
             sA[buf][e / TI][e % TI] = stA[q];
         }
     };
 
     const int nst = (k_end - k_begin + KS - 1) / KS;
-    load_stage(k_begin);
-    store_stage(0);
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        load_piece(k_begin, h);
+        store_piece(0, h);
+    }
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nst) load_stage(k_begin + (st + 1) * KS);
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+        if (st + 1 < nst) load_piece(k_begin + (st + 1) * KS, h);
         __builtin_amdgcn_s_setprio(1);  // the MFMA section: the wave with ready operands issues first
         // scheduling hint: interleave the fragment reads from LDS with the MFMAs
         // (strategy 1; measured ~1 % faster than the default schedule and than strategy 0)
         __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
-        for (int kk = 0; kk < KS; kk += 4) {
+        for (int kk = h * (KS / NH); kk < (h + 1) * (KS / NH); kk += 4) {
             const int kr = kk + lane / 16;
             double are[2], aim[2], asum[2];
 #pragma unroll
@@ -143,7 +153,8 @@ __device__ __forceinline__ void gram_f64_acc(const cplx *__restrict__ bg, const 
             }
         }
         __builtin_amdgcn_s_setprio(0);
-        if (st + 1 < nst) store_stage(buf ^ 1);
+        if (st + 1 < nst) store_piece(buf ^ 1, h);
+        }
         __syncthreads();
     }
 }
@@ -549,7 +560,8 @@ template <int F> struct FuseLds {
     static constexpr int kAq = kBg + kFuseNv * C::TJ;  // [kFuseNv][TI]
     static constexpr int kRed = kAq + kFuseNv * C::TI;  // [WIN = 4][kCols][2F]
     static constexpr int kTotal = kRed + 4 * kCols * 2 * F;
-    static_assert(2 * C::KS * (C::TJ + C::TI) <= kBg, "the Gram stages fit below the occupied-band area");
+    static constexpr int kGram = 2 * C::KS * (C::TJ + C::TI);  // the Gram's stage buffers
+    static constexpr int kLds = kTotal > kGram ? kTotal : kGram;
 };
 
 // Epilogue of one 128 x 128 tile (G-vector tile `itile`) after
