@@ -189,7 +189,7 @@ struct gpp_ctx {
     int ozaki_group = 0;
     int gram_tile = 0;
     int red_chunk = 8192;
-    int red_fc = 8;
+    int red_fc = 0;  // 0: 16 from 16 frequencies up, else 8
     int red_fpb = 0;
     int fuse_reduce = 0;
     int num_cus = 256;
@@ -855,7 +855,8 @@ gpp_status gpp_ctx_set_option(gpp_ctx *ctx, gpp_option option, int value)
         ctx->red_chunk = value;
         return GPP_OK;
     case GPP_OPT_REDUCE_FREQS:
-        if (value != 4 && value != 8) return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs must be 4 or 8, got %d", value);
+        if (value != 0 && value != 4 && value != 8 && value != 16)
+            return fail(GPP_ERR_INVALID_ARGUMENT, "reduce freqs must be 0, 4, 8 or 16, got %d", value);
         ctx->red_fc = value;
         return GPP_OK;
     case GPP_OPT_REDUCE_FREQS_PER_BLOCK:
@@ -954,8 +955,9 @@ gpp_status gpp_ctx_autotune(gpp_ctx *ctx, gpp_form form)
     best = 1e300;
     const int chunks[] = {2048, 4096, 8192, 16384};
     for (int chunk : chunks)
-        for (int fc : {4, 8}) {
-            if (dp.nfreq <= 4 && fc == 8) continue;  // identical launches for nfreq <= 4
+        for (int fc : {4, 8, 16}) {
+            if (dp.nfreq <= 4 && fc != 4) continue;  // identical launches for nfreq <= 4
+            if (dp.nfreq < 16 && fc == 16) continue;  // a 16-wide pass would run idle frequencies
             if ((st = trial(best_tile, chunk, fc, &g, &r))) return st;
             if (r < best) best = r, best_chunk = chunk, best_fc = fc;
         }

@@ -157,13 +157,14 @@ __global__ __launch_bounds__(256) void k_pole_cols(const cplx *col_ach, const cp
 // compile time (0: any).  Three waves per SIMD: with the wave-uniform pole
 // inputs held in scalar registers the FC = 8 variants fit 168 VGPRs (the
 // 8-occupied-band one spills 10), and at 16 frequencies x 8 occupied bands
-// three waves run the reduce 12% faster than two.  Four for the plain
+// three waves run the reduce 12% faster than two.  Two for FC = 16 (its 64
+// accumulators; the plain variants fit 208-241 VGPRs).  Four for the plain
 // kernels of at most three frequencies: their waves wait on memory half of
 // the time, and a fourth hides more of it despite 17 spilled registers
 // (3 frequencies x 2 occupied bands: 0.510 -> 0.486 ms at 512/2/32768/1638);
 // the p2-storing and negative-frequency variants spill 30-44 and keep three.
 template <int FC, int NV, bool P2, bool NEG = false>
-__global__ __launch_bounds__(kRedThreads, (FC <= 3 && !P2 && !NEG) ? 4 : 3) void k_pole_reduce(ReduceArgs a)
+__global__ __launch_bounds__(kRedThreads, (FC <= 3 && !P2 && !NEG) ? 4 : FC > 8 ? 2 : 3) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;
@@ -328,13 +329,17 @@ hipError_t launch_gram_f64(const DevProblem &p, int tile, const cplx *bg, cplx *
     return hipGetLastError();
 }
 
+// red_fc 0: passes of 16 from 16 frequencies up (C3, 16 frequencies: one
+// pass, 7.19 -> 6.74 ms against two of 8), else 8.
 hipError_t launch_pole_reduce(const ReduceArgs &ra, int red_fc, hipStream_t s)
 {
-    switch (ra.nfreq < 4 ? ra.nfreq : (red_fc == 4 ? 4 : 8)) {
+    const int fc = red_fc == 4 || red_fc == 8 || red_fc == 16 ? red_fc : ra.nfreq >= 16 ? 16 : 8;
+    switch (ra.nfreq < 4 ? ra.nfreq : fc) {
     case 1: launch_reduce_fc<1>(ra, s); break;
     case 2: launch_reduce_fc<2>(ra, s); break;
     case 3: launch_reduce_fc<3>(ra, s); break;
     case 4: launch_reduce_fc<4>(ra, s); break;
+    case 16: launch_reduce_fc<16>(ra, s); break;
     default: launch_reduce_fc<8>(ra, s); break;
     }
     return hipGetLastError();

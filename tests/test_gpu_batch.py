@@ -209,7 +209,7 @@ def test_ctx_autotune(ctx, oracle):
     finally:
         ctx.set_option(pygpp.GPP_OPT_GRAM_TILE, 0)
         ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, 8192)
-        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 8)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 0)
 
 
 @pytest.mark.parametrize("form", ["factored-f64", "factored-ozaki-i8"])

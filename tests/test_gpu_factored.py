@@ -252,12 +252,23 @@ def test_ozaki_option_validation(ctx):
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
 
 
+def test_reduce_freqs_option(ctx):
+    """Passes of 4, 8 or 16 frequencies, or 0 (the default): 16 from 16
+    frequencies up, else 8."""
+    import pygpp
+    assert ctx.get_option(pygpp.GPP_OPT_REDUCE_FREQS) == 0
+    for bad in (-1, 1, 12, 32):
+        with pytest.raises(pygpp.GppError):
+            ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, bad)
+    assert ctx.get_option(pygpp.GPP_OPT_REDUCE_FREQS) == 0
+
+
 ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 
-@pytest.mark.parametrize("nfreq", [4, 11, 13])
+@pytest.mark.parametrize("nfreq", [4, 11, 13, 20])
 def test_reduce_schedules_agree(ctx, oracle, nfreq):
-    """Frequencies per pass (4 or 8, the last pass 4 wide when at most 4
+    """Frequencies per pass (4, 8 or 16, the last pass 4 wide when at most 4
     remain) and per block (all, 4, 8) regroup the same per-frequency sums:
     bit-identical for one G-range size; every schedule, at G ranges of 256
     and 8192, within 1e-10 of the oracle."""
@@ -269,7 +280,7 @@ def test_reduce_schedules_agree(ctx, oracle, nfreq):
         for chunk in (256, 8192):
             ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, chunk)
             base = None
-            for fc in (4, 8):
+            for fc in (4, 8, 16):
                 for fpb in (0, 4, 8):
                     ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, fc)
                     ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, fpb)
@@ -283,7 +294,7 @@ def test_reduce_schedules_agree(ctx, oracle, nfreq):
         assert_close(ach, ref, TOL_FACTORED, KEYS + ("acht_n1_loc",))
     finally:
         ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, 8192)
-        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 8)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 0)
         ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, 0)
 
 
