@@ -329,11 +329,12 @@ hipError_t launch_gram_f64(const DevProblem &p, int tile, const cplx *bg, cplx *
     return hipGetLastError();
 }
 
-// red_fc 0: passes of 16 from 16 frequencies up (C3, 16 frequencies: one
-// pass, 7.19 -> 6.74 ms against two of 8), else 8.
+// red_fc 0: passes of 16 from 16 frequencies up, none negative (C3, 16
+// frequencies: one pass, 7.19 -> 6.74 ms against two of 8; the
+// negative-frequency kernel spills at that width), else 8.
 hipError_t launch_pole_reduce(const ReduceArgs &ra, int red_fc, hipStream_t s)
 {
-    const int fc = red_fc == 4 || red_fc == 8 || red_fc == 16 ? red_fc : ra.nfreq >= 16 ? 16 : 8;
+    const int fc = red_fc == 4 || red_fc == 8 || red_fc == 16 ? red_fc : ra.nfreq >= 16 && !ra.neg ? 16 : 8;
     switch (ra.nfreq < 4 ? ra.nfreq : fc) {
     case 1: launch_reduce_fc<1>(ra, s); break;
     case 2: launch_reduce_fc<2>(ra, s); break;

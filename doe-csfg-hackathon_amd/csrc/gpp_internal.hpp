@@ -127,7 +127,8 @@ struct GramKind {
     int group = 0;         // Ozaki G' tiles per tile group (0: sized for the L2)
     int tile = 0;          // FP64 Gram tile: 0 128x128, 1 64x128, 2 128x64, 3 64x64 (G' x G)
     int red_chunk = 8192;  // G-vectors per pole-reduce block
-    int red_fc = 8;        // frequencies per pole-reduce pass when nfreq > 4 (4 or 8)
+    int red_fc = 0;        // frequencies per pole-reduce pass when nfreq > 4 (4, 8, 16; 0: 16 from 16 up, else 8;
+                           // the batched form: 4, else 8)
     int red_fpb = 0;       // frequencies per pole-reduce block (0: all, passes looped in the block)
     bool acht = false;     // also produce acht_n1_loc (a second product, R * aqsn^T)
     bool fuse = false;     // pole reduce in the FP64 Gram epilogue where it applies (no Mu workspace)

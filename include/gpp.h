@@ -119,8 +119,9 @@ typedef enum {
      * (default 8192). */
     GPP_OPT_REDUCE_CHUNK = 4,
     /* Frequencies per pole-reduce pass when nfreq > 4: 4, 8 or 16; 0 = 16
-     * from 16 frequencies up, else 8 (default).  The batched form runs 4 (for
-     * 4) or 8 (otherwise). */
+     * from 16 frequencies up when none is negative, else 8 (default).  The
+     * batched form runs 4 for 4 and 8 otherwise (16-frequency passes timed
+     * no faster on its uneven work lists). */
     GPP_OPT_REDUCE_FREQS = 5,
     /* With band-dependent frequencies (gpp_set_band_frequencies): the most
      * groups of bands sharing one frequency row that the factored forms
