@@ -252,6 +252,30 @@ def test_ozaki_option_validation(ctx):
     assert ctx.get_option(pygpp.GPP_OPT_OZAKI_SLICES) == 6
 
 
+def test_wide_passes_negative_frequencies(ctx, oracle):
+    """20 frequencies, some negative (the ssx cutoff of gppComplex.cpp:58-59
+    live): the automatic pass width keeps 8 there; passes of 16 asked for
+    explicitly run the negative-frequency kernel at that width.  Both within
+    1e-10 of the oracle and bitwise equal (the folding depends on the
+    frequency count only)."""
+    import pygpp
+    arrs = gpp_inputs.seeded_inputs(20, 1500, 60, nfreq=20, seed=67, dw=0.5)
+    arrs["wx_array"] = np.linspace(-2.5, 4.5, 20)
+    ctx.upload(problem_of(arrs, 3), arrs)
+    ref = oracle.sigma(arrs, nvband=3, threads=8)
+    out = {}
+    try:
+        for fc in (0, 8, 16):
+            ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, fc)
+            out[fc] = ctx.sigma("factored-f64")
+            assert_close(out[fc], ref, TOL_FACTORED, KEYS)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 0)
+    for k in KEYS:
+        np.testing.assert_array_equal(out[16][k], out[8][k], err_msg=k)
+        np.testing.assert_array_equal(out[0][k], out[8][k], err_msg=k)
+
+
 def test_reduce_freqs_option(ctx):
     """Passes of 4, 8 or 16 frequencies, or 0 (the default): 16 from 16
     frequencies up, else 8."""
