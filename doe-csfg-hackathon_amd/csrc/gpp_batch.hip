@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(c
         ra.vcoul = b.vcoul; ra.wx = b.wx; ra.nv = b.nv; ra.ncouls = b.ncouls; ra.ngpown = b.ngpown;
         ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
         ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
-        ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1;
+        ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1; ra.part_occ = nullptr;
         ra.fpb = b.nfreq;
         pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC), NEG>(ra, it.j, it.split, 0, b.nfreq, scratch);
     }

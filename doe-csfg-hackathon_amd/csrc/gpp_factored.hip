@@ -12,11 +12,14 @@
 //   4. k_fact_final    fixed-order column reduction; achstemp (reduce_achstemp,
 //                      gppComplex.cpp:66-101) from k_achstemp.
 // When acht_n1_loc is requested (GramKind::acht), stage 3 also writes the
-// pole factor p2 of frequency 2 (and the occupied branch mask), and
+// pole factor p2 of frequency 2 and, with two occupied bands (and no negative
+// frequency), sums the occupied bands' terms (the masked p2 against their
+// aqsn rows, already loaded for Mo); and
 //   5. k_acht_f64      second FP64 MFMA product Y = P2 * aqsn^T for the
 //                      unoccupied bands, split over G-vectors, reduced per
-//                      band in its epilogue; k_acht_occ streams the masked
-//                      P2 against the few occupied aqsn rows
+//                      band in its epilogue; for the other occupied band
+//                      counts k_acht_occ streams the masked P2 (stage 3
+//                      writes the branch mask) against the occupied aqsn rows
 //   6. k_acht_final    fixed-order sum of the per-band partials.
 #include <string.h>
 
@@ -50,7 +53,7 @@ AchtSplit acht_split(const DevProblem &p)
     int nks = (4096 + tiles - 1) / tiles;
     const int max_nks = p.ncouls / 512 > 1 ? p.ncouls / 512 : 1;
     if (nks > max_nks) nks = max_nks;
-    a.chunk = ((p.ncouls + nks - 1) / nks + kAK - 1) / kAK * kAK;
+    a.chunk = ((p.ncouls + nks - 1) / nks + kAKS - 1) / kAKS * kAKS;
     a.nks = (p.ncouls + a.chunk - 1) / a.chunk;
     return a;
 }
@@ -94,7 +97,9 @@ FactLayout fact_layout(const DevProblem &p, const GramKind &k)
     if (k.acht) {
         L.as = acht_split(p);
         L.p2 = off;        off += align256(sizeof(cplx) * (size_t)p.ngpown * p.ncouls);
-        L.br1 = off;       off += p.nv > 0 ? align256((size_t)p.ngpown * p.ncouls) : 0;
+        // branch-1 mask for k_acht_occ, where the reduce does not sum the
+        // occupied bands itself
+        L.br1 = off;       off += p.nv > 0 && !reduce_sums_acht_occ(p.nv, p.wx_neg) ? align256((size_t)p.ngpown * p.ncouls) : 0;
         L.acht_part = off; off += align256(sizeof(cplx) * (size_t)L.as.jtiles * L.as.nks * p.nb);
         L.acht_occ = off;  off += align256(sizeof(cplx) * (size_t)p.ngpown * L.nsplit * (p.nv > 0 ? p.nv : 1));
     }
@@ -194,8 +199,8 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restr
                                                                     cplx *__restrict__ part)
 {
     typedef AchtCfg<WJN, WIN> C;
-    __shared__ cplx sR[2][C::TJ][kAK];
-    __shared__ cplx sA[2][C::TN][kAK];
+    __shared__ cplx sR[2][2][C::TJ][kAK];
+    __shared__ cplx sA[2][2][C::TN][kAK];
     __shared__ cplx red[WJN][C::TN];
     const int nn = (n1_end - n1_begin + C::TN - 1) / C::TN;
     const int t = xcd_tile(blockIdx.x, nn * jtiles * nks);
@@ -207,7 +212,8 @@ __global__ __launch_bounds__(64 * WJN * WIN) void k_acht_f64(const cplx *__restr
                                   part + ((size_t)jt * nks + ks) * nb);
 }
 
-// Grid (ngpown, nsplit): the occupied bands' acht_n1_loc partials.
+// Grid (ngpown, nsplit): the occupied bands' acht_n1_loc partials (where the
+// reduce does not sum them: reduce_sums_acht_occ).
 __global__ __launch_bounds__(kRedThreads) void k_acht_occ(const cplx *__restrict__ p2,
                                                           const uint8_t *__restrict__ br1,
                                                           const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
@@ -266,8 +272,10 @@ void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t 
                        p.nb, mu);
 }
 
-// P2: also store the pole factor (and occupied branch) of frequency acht_iw.
-// Problems with a negative frequency run the generic NEG kernel.
+// P2: also store the pole factor of frequency acht_iw and either sum the
+// occupied bands' acht_n1_loc terms (reduce_sums_acht_occ) or store the
+// occupied branch mask.  Problems with a negative frequency run the generic
+// NEG kernel.
 template <int FC, bool P2>
 void launch_reduce_p2(const ReduceArgs &a, dim3 grid, hipStream_t s)
 {
@@ -399,7 +407,9 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
     ra.nsplit = L.nsplit; ra.chunk = kind.red_chunk;
     ra.fpb = kind.red_fpb > 0 ? kind.red_fpb : p.nfreq;
     ra.p2 = kind.acht ? reinterpret_cast<cplx *>(w + L.p2) : nullptr;
-    ra.br1 = kind.acht && p.nv > 0 ? reinterpret_cast<uint8_t *>(w + L.br1) : nullptr;
+    const bool occ_in_reduce = kind.acht && p.nv > 0 && reduce_sums_acht_occ(p.nv, p.wx_neg);
+    ra.br1 = kind.acht && p.nv > 0 && !occ_in_reduce ? reinterpret_cast<uint8_t *>(w + L.br1) : nullptr;
+    ra.part_occ = occ_in_reduce ? reinterpret_cast<cplx *>(w + L.acht_occ) : nullptr;
     ra.acht_iw = p.nfreq > 2 ? 2 : p.nfreq - 1;  // sch_array[2], gppComplex.cpp:381
     ra.neg = p.wx_neg;
     GPP_TRY(launch_pole_reduce(ra, kind.red_fc, s));
@@ -418,7 +428,7 @@ hipError_t run_factored(const DevProblem &p, GramKind kind, void *work, const De
                                p.vcoul, p.ngpown, p.ncouls, p.nb, p.nv, p.nb, as.chunk, as.nks, as.jtiles, part);
             GPP_TRY(hipGetLastError());
         }
-        if (p.nv > 0) {
+        if (p.nv > 0 && !occ_in_reduce) {  // else summed by the reduce
             hipLaunchKernelGGL(k_acht_occ, dim3(p.ngpown, L.nsplit), dim3(kRedThreads), 0, s, ra.p2, ra.br1, p.aqsn,
                                bg, p.idx, p.vcoul, p.ngpown, p.ncouls, p.nv, kind.red_chunk, L.nsplit, part_occ);
             GPP_TRY(hipGetLastError());

@@ -241,7 +241,16 @@ struct ReduceArgs {
     uint8_t *br1;
     int acht_iw;
     int neg;  // some frequency is negative: launch the NEG variant (host-side dispatch only)
+    // With p2 and reduce_sums_acht_occ: the occupied bands' acht_n1_loc
+    // partials, [ngpown][nsplit][nv], summed in the pass (br1 not written).
+    cplx *part_occ;
 };
+
+// Whether the p2-storing reduce sums the occupied bands' acht_n1_loc terms
+// itself or leaves them to acht_occ_item (br1): two occupied bands, whose
+// aqsn values and sums fit beside the pass (with 4 or 8 the kernels spill,
+// at 8 and 16 frequencies by 45-297 registers).
+__host__ __device__ inline bool reduce_sums_acht_occ(int nv, int neg) { return !neg && nv == 2; }
 
 // Stage 3: pole terms of column j over G-vectors [split*chunk, +chunk), for
 // frequencies [iw0, iw0+FC).  Per element and frequency: wdiff, |wdiff|^2,
@@ -257,7 +266,10 @@ struct ReduceArgs {
 // column is loaded once; NV == 0: any a.nv.  NEG: some frequency may be
 // negative, where the ssx cutoff of :58-59 is live; without it every
 // frequency must be >= 0 (the launcher checks on the host) and the kernel
-// carries no cutoff code.
+// carries no cutoff code.  P2 with NV == 2 (and no NEG): the pass holding
+// frequency acht_iw also sums sum_ig aqsn[n1,ig] p2[j,ig] [branch 1] per
+// occupied band n1 (the same products in the same order as acht_occ_item,
+// whose partials it writes to part_occ).
 template <int FC, int NV = 0, bool P2 = false, int FCT = FC, bool NEG = false>
 __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int split, int iw_begin, int iw_end,
                                                  cplx *scratch)
@@ -265,6 +277,7 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
 #pragma clang fp contract(off)
     const int ncouls = a.ncouls;
     const int nv = NV > 0 ? NV : a.nv;
+    constexpr bool kOccSum = P2 && NV == 2 && !NEG;  // reduce_sums_acht_occ
 
     const int ig_begin = split * a.chunk;
     const int ig_end = ig_begin + a.chunk < ncouls ? ig_begin + a.chunk : ncouls;
@@ -294,6 +307,9 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         cplx sach[F], sasx[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) sach[f] = sasx[f] = czero();
+        cplx oacc[kOccSum ? NV : 1];  // occupied acht_n1_loc sums (kOccSum)
+#pragma unroll
+        for (int n1 = 0; n1 < (kOccSum ? NV : 1); ++n1) oacc[n1] = czero();
 
         bool any_neg = false;  // the ssx cutoff of :58-59 applies only at negative frequencies
         if constexpr (NEG) {
@@ -341,22 +357,23 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 u0_im_wti = u0.im * wt.im;
                 u0_re_wti = u0.re * wt.im;
             }
+            cplx av[NV > 0 ? NV : 1];  // the occupied aqsn of this G-vector, loads issued together
             if (nv > 0) {
                 // Mo over the occupied bands, in band order: folded passes
                 // take four FMAs per band (C3 reduce 7.54 -> 7.33 ms, C4 14.39
                 // -> 14.14 ms), the three-frequency kernel a product and a sum
                 // (0.487 against 0.501 ms at 512/2/32768/1638)
-                if constexpr (NV > 0 && FOLD) {
-                    cplx av[NV > 0 ? NV : 1];  // loads issued together
+                if constexpr (NV > 0) {
 #pragma unroll
                     for (int n1 = 0; n1 < NV; ++n1) av[n1] = a.aqsn[(size_t)n1 * ncouls + ig];
+                }
+                if constexpr (NV > 0 && FOLD) {
                     mo = cmul_fast(bgc[0], av[0]);
 #pragma unroll
                     for (int n1 = 1; n1 < NV; ++n1) mo = cfma(bgc[n1], av[n1], mo);
                 } else if constexpr (NV > 0) {
 #pragma unroll
-                    for (int n1 = 0; n1 < NV; ++n1)
-                        mo = cadd(mo, cmul_fast(bgc[n1], a.aqsn[(size_t)n1 * ncouls + ig]));
+                    for (int n1 = 0; n1 < NV; ++n1) mo = cadd(mo, cmul_fast(bgc[n1], av[n1]));
                 } else if constexpr (FOLD) {
                     for (int n1 = 0; n1 < nv; ++n1)
                         mo = cfma(cconj(a.bg[(size_t)n1 * a.ngpown + j]), a.aqsn[(size_t)n1 * ncouls + ig], mo);
@@ -455,9 +472,17 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                     sasx[f].im = __builtin_fma(rr, __builtin_fma(uo.im, s, -uo_re_wti), sasx[f].im);
                 }
                 if (P2 && iw0 + f == a.acht_iw && iw0 + f < iw_end) {
-                    a.p2[row + ig] = cplx{__builtin_fma(q.re, wdr, -(FOLD ? q.im * wt.im : q_im_wti_e)) * rden,
-                                          __builtin_fma(q.im, wdr, FOLD ? q.re * wt.im : q_re_wti_e) * rden};
-                    if (a.br1) a.br1[row + ig] = branch1 ? 1 : 0;
+                    const cplx pv = {__builtin_fma(q.re, wdr, -(FOLD ? q.im * wt.im : q_im_wti_e)) * rden,
+                                     __builtin_fma(q.im, wdr, FOLD ? q.re * wt.im : q_re_wti_e) * rden};
+                    a.p2[row + ig] = pv;
+                    if constexpr (kOccSum) {
+                        if (branch1) {  // sch = P (:37)
+#pragma unroll
+                            for (int n1 = 0; n1 < (kOccSum ? NV : 1); ++n1) oacc[n1] = cfma(av[n1], pv, oacc[n1]);
+                        }
+                    } else if (a.br1) {
+                        a.br1[row + ig] = branch1 ? 1 : 0;
+                    }
                 }
             }
         }
@@ -471,6 +496,18 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
                 const size_t o = ((size_t)j * a.nsplit + split) * a.nfreq + iw0 + f;
                 a.col_ach[o] = cscale(cscale(s, 0.5), vc);
                 a.col_asx[o] = cscale(x, a.occ * vc);
+            }
+        }
+        if constexpr (kOccSum) {
+            if (a.acht_iw >= iw0 && a.acht_iw < iw0 + F && a.acht_iw < iw_end) {  // uniform
+                // 0.5 vcoul[igp(j)] conj(aqsm[n1,igp(j)]) sum, gppComplex.cpp:381
+                const double w = 0.5 * vc;
+#pragma unroll
+                for (int n1 = 0; n1 < (kOccSum ? NV : 1); ++n1) {
+                    const cplx s = block_sum(oacc[n1], scratch);
+                    if (threadIdx.x == 0)
+                        a.part_occ[((size_t)j * a.nsplit + split) * NV + n1] = cscale(cmul_fast(bgc[n1], s), w);
+                }
             }
         }
     };
@@ -741,8 +778,11 @@ __device__ __forceinline__ void pole_cols_wave(const cplx *col_ach, const cplx *
 // of 8 complex, element (row, k) at slot k ^ (row & 7): the stores (8 lanes
 // per row) and the MFMA reads (16 rows per 16 lanes) are then free of LDS
 // bank conflicts on gfx950 (measured with SQ_LDS_BANK_CONFLICT; a padded
-// pitch of 9 complex is not).
-constexpr int kAK = 8;  // G-vectors per LDS stage of the acht product
+// pitch of 9 complex is not).  A stage is two such pieces of 8 G-vectors
+// (one barrier per 16), each loaded through registers during the previous
+// piece's MFMAs and stored to the other buffer right after them.
+constexpr int kAK = 8;          // G-vectors per LDS piece of the acht product
+constexpr int kAKS = 2 * kAK;   // per stage
 
 template <int WJN, int WIN> struct AchtCfg {
     static constexpr int TJ = kWJ * WJN, TN = kWI * WIN, NT = 64 * WJN * WIN;
@@ -753,8 +793,8 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
                                               const cplx *__restrict__ aqsn, const cplx *__restrict__ bg,
                                               const int32_t *__restrict__ idx, const double *__restrict__ vcoul,
                                               int ngpown, int ncouls, int n1_end, int k_begin, int k_end, int j0,
-                                              int c0, cplx (*sR)[AchtCfg<WJN, WIN>::TJ][kAK],
-                                              cplx (*sA)[AchtCfg<WJN, WIN>::TN][kAK],
+                                              int c0, cplx (*sR)[2][AchtCfg<WJN, WIN>::TJ][kAK],
+                                              cplx (*sA)[2][AchtCfg<WJN, WIN>::TN][kAK],
                                               cplx (*red)[AchtCfg<WJN, WIN>::TN], cplx *__restrict__ part)
 {
     typedef AchtCfg<WJN, WIN> C;
@@ -774,7 +814,7 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
         for (int t = 0; t < 2; ++t) acc1[s][t] = acc2[s][t] = acc3[s][t] = v4d{0, 0, 0, 0};
 
     cplx stR[PR], stA[PA];
-    auto load_stage = [&](int k0) {
+    auto load_piece = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < PR; ++q) {
             const int e = threadIdx.x + q * NT;
@@ -788,42 +828,47 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
             stA[q] = (k < k_end && n1 < n1_end) ? aqsn[(size_t)n1 * ncouls + k] : czero();
         }
     };
-    auto store_stage = [&](int buf) {
+    auto store_piece = [&](int buf, int h) {
 #pragma unroll
         for (int q = 0; q < PR; ++q) {
             const int e = threadIdx.x + q * NT;
             const int r = e / kAK;
-            sR[buf][r][(e % kAK) ^ (r & 7)] = stR[q];
+            sR[buf][h][r][(e % kAK) ^ (r & 7)] = stR[q];
         }
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
             const int e = threadIdx.x + q * NT;
             const int r = e / kAK;
-            sA[buf][r][(e % kAK) ^ (r & 7)] = stA[q];
+            sA[buf][h][r][(e % kAK) ^ (r & 7)] = stA[q];
         }
     };
 
-    const int nst = (k_end - k_begin + kAK - 1) / kAK;
-    load_stage(k_begin);
-    store_stage(0);
+    const int nst = (k_end - k_begin + kAKS - 1) / kAKS;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        load_piece(k_begin + h * kAK);
+        store_piece(0, h);
+    }
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nst) load_stage(k_begin + (st + 1) * kAK);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+        if (st + 1 < nst) load_piece(k_begin + (st + 1) * kAKS + h * kAK);
 #pragma unroll
         for (int kk = 0; kk < kAK; kk += 4) {
             const int kr = (kk + lane / 16) ^ (lane & 7);  // every row read below is = lane (mod 8)
             double are[2], aim[2], asum[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const cplx a = sA[buf][wi + 16 * t + lane % 16][kr];
+                const cplx a = sA[buf][h][wi + 16 * t + lane % 16][kr];
                 are[t] = a.re;
                 aim[t] = a.im;
                 asum[t] = a.re + a.im;
             }
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const cplx r = sR[buf][wj + 16 * s + lane % 16][kr];
+                const cplx r = sR[buf][h][wj + 16 * s + lane % 16][kr];
                 const double rsum = r.re + r.im;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
@@ -833,7 +878,8 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
                 }
             }
         }
-        if (st + 1 < nst) store_stage(buf ^ 1);
+        if (st + 1 < nst) store_piece(buf ^ 1, h);
+        }
         __syncthreads();
     }
 

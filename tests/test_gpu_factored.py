@@ -276,6 +276,42 @@ def test_wide_passes_negative_frequencies(ctx, oracle):
         np.testing.assert_array_equal(out[0][k], out[8][k], err_msg=k)
 
 
+def test_acht_two_occupied_bands_schedules(ctx, oracle):
+    """acht_n1_loc with two occupied bands, whose terms the p2-storing reduce
+    sums itself (the other counts stream the branch mask through
+    k_acht_occ): against the oracle with 12 G-vector splits per column and
+    one, over pass and block widths that put frequency 2 in passes of 4, 8
+    and 16 and in blocks of 4 and 8 frequencies, bit-identical across the
+    widths within a chunk; then with a negative frequency (the generic
+    kernel, mask and k_acht_occ)."""
+    import pygpp
+    keys = KEYS + ("acht_n1_loc",)
+    arrs = gpp_inputs.seeded_inputs(24, 3000, 40, nfreq=20, seed=91, dw=0.3)
+    ctx.upload(problem_of(arrs, 2), arrs)
+    ref = oracle.sigma(arrs, nvband=2, threads=8)
+    try:
+        for chunk in (256, 8192):
+            ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, chunk)
+            base = None
+            for fc, fpb in ((4, 0), (8, 0), (16, 0), (8, 4), (4, 8)):
+                ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, fc)
+                ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, fpb)
+                out = ctx.sigma("factored-f64", acht_n1_loc=True)
+                assert_close(out, ref, TOL_FACTORED, keys)
+                if base is None:
+                    base = out
+                for k in keys:
+                    np.testing.assert_array_equal(out[k], base[k], err_msg=f"{chunk} {fc} {fpb} {k}")
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_CHUNK, 8192)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 0)
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS_PER_BLOCK, 0)
+    arrs["wx_array"] = np.linspace(-1.5, 3.5, 20)
+    ctx.upload(problem_of(arrs, 2), arrs)
+    assert_close(ctx.sigma("factored-f64", acht_n1_loc=True), oracle.sigma(arrs, nvband=2, threads=8),
+                 TOL_FACTORED, keys)
+
+
 def test_reduce_freqs_option(ctx):
     """Passes of 4, 8 or 16 frequencies, or 0 (the default): 16 from 16
     frequencies up, else 8."""
