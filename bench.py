@@ -347,12 +347,17 @@ def main():
     # "factored-f64-fused" is factored-f64 with the pole reduce in the Gram
     # epilogue (GPP_OPT_FUSED_REDUCE = 1, at most three frequencies): no G'xG
     # workspace, slower where that fits.
+    # "...@5": the Ozaki form at 5 slices (39-bit digits) instead of the
+    # default 6 (47 bits for INT8, 42 for FP16): 15 digit pairs instead of 21.
     extra = ["factored-f64+acht"] + (["factored-f64-fused"] if NFREQ <= 3 else [])
+    extra += ["factored-ozaki-i8@5"] + (["factored-ozaki@5"] if args.workload == "c1" else [])
     for form in run_forms + extra:
         name, acht = (form[:-5], True) if form.endswith("+acht") else (form, None)
         fused = name.endswith("-fused")
         if fused:
             name = name[:-len("-fused")]
+        slices = 5 if name.endswith("@5") else 6
+        name = name[:-2] if name.endswith("@5") else name
         # the direct forms (50 ms per step at c1, 3 s at c3) run at their
         # sustained clock from the first step: a few steps and one warmup
         if form == args.form:
@@ -362,7 +367,9 @@ def main():
         else:
             steps, warm = max(1, args.steps // 2), args.warmup
         ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, 1 if fused else 0)
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, slices)
         out, ms, stages = time_form(ctx, name, steps, warm, acht)
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, 6)
         ctx.set_option(pygpp.GPP_OPT_FUSED_REDUCE, 0)
         results[form] = out
         forms[form] = {"ms_per_step": ms, "updates_per_s": updates / (ms * 1e-3), "gpu_stage_ms": stages}
@@ -415,9 +422,11 @@ def main():
                 pname += "; the pole reduce of its elements in the epilogue, timed with it"
             tr = load_traffic("k_gram_pole" if fused else "k_gram_f64") if args.workload == "c1" else None
         else:
-            # executed Ozaki work: 21 digit pairs x 4 real products, split included in the stage time
-            flop = 21 * 4 * 2 * NGPOWN * NCOULS * K
-            i8 = form.endswith("i8")
+            # executed Ozaki work: S (S + 1) / 2 digit pairs x 4 real products, split included in the
+            # stage time
+            S = 5 if form.endswith("@5") else 6
+            flop = S * (S + 1) // 2 * 4 * 2 * NGPOWN * NCOULS * K
+            i8 = "-i8" in form
             peak = PEAK_INT8_TOPS if i8 else PEAK_FP16_TFLOPS
             # the default block layout (4, persistent layout 3) runs 32x32 MFMAs
             pname = ("v_mfma_i32_32x32x32_i8" if i8 else "v_mfma_f32_32x32x16_f16") + ", digit splits included"
@@ -425,7 +434,7 @@ def main():
         achieved = flop / (gms * 1e-3) / 1e12
         extra = {"zgemm_equivalent_tflops": achieved * 8 / F64_FLOP_PER_CMAC} if form.startswith("factored-f64") else {}
         line = {**extra, "bound": "mfma", "kernel": f"{gkey} ({pname})", "achieved": achieved,
-                "peak": peak, "unit": "TFLOP/s" if not form.endswith("i8") else "TOP/s", "frac": achieved / peak,
+                "peak": peak, "unit": "TOP/s" if "-i8" in form else "TFLOP/s", "frac": achieved / peak,
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
                 "algorithmic_flop_per_launch": flop, "avg_launch_ms": gms}
         if fused:

@@ -4,30 +4,32 @@
 //
 //   Mu[j][ig] = sum_{k in [nv, nb)} conj(Bg[k][j]) * aqsn[k][ig]
 //
-// Splitting (error-free).  Every column c of an operand (a G' column j of Bg,
-// a G-vector ig of aqsn) gets a power-of-two scale 2^e_c > max_k |re|, |im|.
-// Each real part x is written as
-//   x = 2^e_c * sum_{p<S} d_p 2^{-6-7p} + r,   d_p integer in [-64, 64],
-//   |r| <= 2^(e_c - 7S)
-// (d_0 = rint(x 2^(6-e_c)), each later digit rounds 2^7 times the previous
-// remainder).  Digits are exact in FP16.  The product of digits p and q has
-// weight 2^(e_j + e_ig - 12 - 7(p+q)); all pairs with the same level t = p+q
-// share one FP32 accumulator, and pairs with t >= S are dropped (their weight
-// is below the split's own truncation).
+// Splitting.  Every column c of an operand (a G' column j of Bg, a G-vector
+// ig of aqsn) gets a power-of-two scale 2^e_c > max_k |re|, |im|.  Each real
+// part x is written as
+//   x = 2^(e_c - 6) * sum_{p<S} d_p B^-p + r,   d_0 in [-64, 64],
+//   d_p in [-B/2, B/2] (p >= 1),   |r| <= 2^(e_c - 7) B^(1-S)
+// (d_0 = rint(x 2^(6-e_c)), each later digit rounds B times the previous
+// remainder).  B = 254: digits of about 8 bits after the first, exact in FP16
+// and int8, 39 bits at S = 5 and 47 at S = 6; FP16 at S = 6 uses B = 128 (42
+// bits; OzAcc in gpp_ozaki_dev.hpp).  The product of digits p and q has weight
+// 2^(e_j + e_ig - 12) B^-(p+q); the pairs of one level t = p+q share an
+// accumulator, and pairs with t >= S are dropped (their weight is below the
+// split's own truncation).  Levels fold to FP64 by Horner in 1/B.
 //
-// Exactness.  A level-t accumulator receives (t+1) pairs x 2 real products
-// (conj(b)a: br ar + bi ai, br ai - bi ar) of magnitude <= 64*64 per band, so
-// over a chunk of 256 bands |sum| <= 6 * 2 * 4096 * 256 < 2^24 for
-// S <= 6: every partial sum is an integer below 2^24 and FP32 holds it
-// exactly, whatever order the MFMA adds in.  After each chunk the levels are
-// converted to FP64 and accumulated.
+// Accumulation.  A level-t accumulator receives (t+1) pairs x 2 real products
+// (conj(b)a: br ar + bi ai, br ai - bi ar) per band.  FP16 digits accumulate
+// in FP32 over chunks of 256 bands, exactly (integers below 2^24, which FP32
+// holds whatever order the MFMA adds in) for every level with B = 128 and for
+// levels 0-3 with B = 254 (level 3 in two accumulators); levels 4 and up of
+// the latter round, within 2^-40 of the chunk's largest possible sum.  After
+// each chunk the levels are converted to FP64 and accumulated.
 //
-// The same kernel runs with INT8 digits (v_mfma_i32_16x16x64_i8): digits in
-// [-64, 64] are exact int8, products accumulate exactly in INT32 over chunks
-// of 32768 bands (6 * 2 * 4096 * 32768 < 2^31), so up to that band count the
-// levels are folded to FP64 once, after the whole band range, and longer
-// ranges are flushed per chunk; MI355X runs INT8 MFMA at twice the FP16 rate
-// and the staged digits are half the bytes.
+// The same kernel runs with INT8 digits (v_mfma_i32_16x16x64_i8): INT32 sums
+// are exact over chunks of 8192 bands (at most 161544 per band and level at
+// S = 6), so up to that band count the levels are folded to FP64 once, after
+// the whole band range, and longer ranges are flushed per chunk; MI355X runs
+// INT8 MFMA at twice the FP16 rate and the staged digits are half the bytes.
 //
 // MFMA lane layouts (measured on gfx950), 16 bytes of operand per lane:
 //   f32_16x16x32_f16: A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]
@@ -134,21 +136,27 @@ void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad, const int 
     // sums (FP16) the FP64 running sums of the chunks are kept, unscaled, in
     // the thread's own output elements of mout (registers are the limit here)
     // and scaled after the last chunk.
-    acc_t acc_re[S][SJ][SI], acc_im[S][SJ][SI];
+    typedef OzAcc<D, S> A;
+    acc_t acc_re[A::N][SJ][SI], acc_im[A::N][SJ][SI];
 #pragma unroll
     for (int sj = 0; sj < SJ; ++sj)
 #pragma unroll
         for (int si = 0; si < SI; ++si)
 #pragma unroll
-            for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
+            for (int t = 0; t < A::N; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
     // Horner over the levels (smallest first) of result r of sub-tile (sj, si).
     auto fold = [&](int sj, int si, int r, double &sr, double &sm) {
         sr = 0.0;
         sm = 0.0;
 #pragma unroll
         for (int t = S - 1; t >= 0; --t) {
-            sr = sr * 0x1p-7 + (double)acc_re[t][sj][si][r];
-            sm = sm * 0x1p-7 + (double)acc_im[t][sj][si][r];
+            double vr = (double)acc_re[A::first(t)][sj][si][r], vm = (double)acc_im[A::first(t)][sj][si][r];
+            if (A::SPLIT && t == 3) {  // the level's second half (exact sum: both below 2^24)
+                vr += (double)acc_re[A::SPLIT ? 4 : 0][sj][si][r];
+                vm += (double)acc_im[A::SPLIT ? 4 : 0][sj][si][r];
+            }
+            sr = sr * A::INV_BASE + vr;
+            sm = sm * A::INV_BASE + vm;
         }
     };
 
@@ -185,7 +193,7 @@ void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad, const int 
 #pragma unroll
             for (int si = 0; si < SI; ++si)
 #pragma unroll
-                for (int t = 0; t < S; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
+                for (int t = 0; t < A::N; ++t) acc_re[t][sj][si] = acc_im[t][sj][si] = acc_t{0, 0, 0, 0};
     };
 
     // Digit planes in HBM: 16-byte pieces (KP bands of one column), pieces of
@@ -265,7 +273,7 @@ void k_gram_ozaki(const D *__restrict__ bd, const D *__restrict__ ad, const int 
             }
 #pragma unroll
             for (int q = 0; q + p < S; ++q) {
-                const int t = p + q;
+                const int t = A::idx(p, q);
                 frag fa[2][SI];  // [real, imaginary part][sub-tile] of digit q
 #pragma unroll
                 for (int part = 0; part < 2; ++part)

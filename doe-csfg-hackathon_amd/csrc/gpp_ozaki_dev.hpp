@@ -19,14 +19,16 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kOzTile = 64;  // padding of both operands' columns
 
-// Digit-type traits: FP16 digits on f32_16x16x32_f16 with exact FP32 sums over
-// chunks of 256 bands; INT8 digits on i32_16x16x64_i8 with exact INT32 sums.
+// Digit-type traits: FP16 digits on f32_16x16x32_f16 with FP32 sums over
+// chunks of 256 bands (exact for levels 0-2); INT8 digits on i32_16x16x64_i8
+// with exact INT32 sums.
 template <typename D> struct OzDigit;
 template <> struct OzDigit<_Float16> {
     typedef v8h frag;
     typedef v4f acc;
     static constexpr int KS = 32;       // bands per MFMA k-step = per LDS stage (64 bytes)
-    static constexpr int CHUNK = 256;   // bands per exact accumulation chunk
+    // Bands per FP32 accumulation chunk (OzAcc: the levels exact over it).
+    static constexpr int CHUNK = 256;
     static constexpr bool ONE_CHUNK = false;
     static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
     {
@@ -50,11 +52,12 @@ template <> struct OzDigit<int8_t> {
     typedef v4i frag;
     typedef v4i acc;
     static constexpr int KS = 64;
-    // INT32 sums are exact over 32768 bands (6 * 2 * 4096 * 32768 < 2^31):
+    // INT32 sums are exact over 8192 bands (a level takes at most
+    // 2 (2*64*127 + 4*127^2) = 161544 per band at 6 slices; x 8192 < 2^31):
     // band ranges up to that are one chunk, folded to FP64 once at the end;
     // longer ones run the tile instantiated for long ranges (LONG), which
-    // flushes to FP64 every 32768 bands like the FP16 chunks.
-    static constexpr int CHUNK = 32768;
+    // flushes to FP64 every 8192 bands like the FP16 chunks.
+    static constexpr int CHUNK = 8192;
     static constexpr bool ONE_CHUNK = true;
     static __device__ __forceinline__ acc mfma(frag a, frag b, acc c)
     {
@@ -78,6 +81,39 @@ template <> struct OzDigit<int8_t> {
         }
         return r;
     }
+};
+
+// Digits and accumulators of S slices of digit type D.
+//
+// Digits: a real part x of a column with exponent e is
+//   x = 2^(e-6) (d_0 + d_1 / B + d_2 / B^2 + ...),
+// d_0 in [-64, 64] (x 2^(6-e) rounded), d_p for p >= 1 in [-B/2, B/2] (B times
+// the previous remainder, at most 0.5 in magnitude, rounded).  B = 254 (about
+// 8 bits per digit after the first: 39 bits at 5 slices, 47 at 6; exact in
+// int8 and FP16), except FP16 at 6 slices, which keeps B = 128 (42 bits): its
+// wider digits would need a fourteenth accumulator, beyond the registers.
+//
+// Accumulators: one per level t = p + q of the digit pairs (p, q), folded
+// with weight B^-t.  Per band a level takes (t+1) pairs x 2 products.  INT8
+// sums are exact in INT32 over a chunk.  FP16 sums, in FP32, are exact while
+// below 2^24 -- over 256 bands: with B = 128 every level (6 * 2 * 64^2 * 256
+// < 2^24); with B = 254 levels 0-2 (level 2: 2 (2*64*127 + 127^2) * 256 =
+// 16581120) and level 3 split over two accumulators (pairs p <= 1 and p >= 2,
+// each 2 (64*127 + 127^2) * 256 = 12419584), while levels 4 and up round,
+// within 2^-40 of the chunk's largest possible sum given their weights
+// (254^-4 and below, half an ulp of 2^26 per product at most).
+template <typename D, int S> struct OzAcc {
+    static constexpr bool WIDE = !(sizeof(D) == 2 && S == 6);  // B = 254
+    static constexpr double BASE = WIDE ? 254.0 : 128.0;
+    static constexpr double INV_BASE = 1.0 / BASE;
+    static constexpr bool SPLIT = sizeof(D) == 2 && WIDE && S >= 4;
+    static constexpr int N = SPLIT ? S + 1 : S;  // accumulators
+    static constexpr __host__ __device__ int idx(int p, int q)
+    {
+        return !SPLIT || p + q < 3 ? p + q : p + q == 3 ? (p < 2 ? 3 : 4) : p + q + 1;
+    }
+    // accumulator of level t (its first half where split)
+    static constexpr __host__ __device__ int first(int t) { return !SPLIT || t <= 3 ? t : t + 1; }
 };
 
 // 16 bytes per lane from global memory straight into LDS: the wave writes
@@ -156,8 +192,11 @@ __device__ __forceinline__ void split_piece(const cplx *__restrict__ src, int ld
             const double dr = rint(yr), di = rint(yi);
             dre[p][kk] = (D)dr;
             dim[p][kk] = (D)di;
-            yr = (yr - dr) * 128.0;  // exact: |yr - dr| <= 0.5
-            yi = (yi - di) * 128.0;
+            // |yr - dr| <= 0.5, so the next digit is in [-B/2, B/2]; with
+            // B = 254 the product's rounding (2^-53 of the remainder) is far
+            // below the split's truncation (with B = 128 it is exact)
+            yr = (yr - dr) * OzAcc<D, S>::BASE;
+            yi = (yi - di) * OzAcc<D, S>::BASE;
         }
     }
     const size_t npieces = (size_t)(kpad / KP);
@@ -260,9 +299,10 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
         }
     };
 
-    acc_t acc_re[S], acc_im[S];
+    typedef OzAcc<D, S> A;
+    acc_t acc_re[A::N], acc_im[A::N];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
+    for (int t = 0; t < A::N; ++t) {
         acc_re[t] = acc_t{};
         acc_im[t] = acc_t{};
     }
@@ -274,8 +314,13 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
         sm = 0.0;
 #pragma unroll
         for (int t = S - 1; t >= 0; --t) {
-            sr = sr * 0x1p-7 + (double)acc_re[t][r];
-            sm = sm * 0x1p-7 + (double)acc_im[t][r];
+            double vr = (double)acc_re[A::first(t)][r], vm = (double)acc_im[A::first(t)][r];
+            if (A::SPLIT && t == 3) {  // the level's second half (exact sum: both below 2^24)
+                vr += (double)acc_re[A::SPLIT ? 4 : 0][r];
+                vm += (double)acc_im[A::SPLIT ? 4 : 0][r];
+            }
+            sr = sr * A::INV_BASE + vr;
+            sm = sm * A::INV_BASE + vm;
         }
     };
     // Column exponents of this lane's 16 results (eb, ea hold the padded
@@ -310,7 +355,7 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
             }
         }
 #pragma unroll
-        for (int t = 0; t < S; ++t) {
+        for (int t = 0; t < A::N; ++t) {
             acc_re[t] = acc_t{};
             acc_im[t] = acc_t{};
         }
@@ -357,14 +402,14 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
             // products into one accumulator are issued S - p MFMAs apart.
 #pragma unroll
             for (int q = 0; q + p < S; ++q) {
-                acc_re[p + q] = T::mfma32(br, fa[q][0], acc_re[p + q]);
-                acc_im[p + q] = T::mfma32(br, fa[q][1], acc_im[p + q]);
+                acc_re[A::idx(p, q)] = T::mfma32(br, fa[q][0], acc_re[A::idx(p, q)]);
+                acc_im[A::idx(p, q)] = T::mfma32(br, fa[q][1], acc_im[A::idx(p, q)]);
             }
             if (prefetch) issue_stage(st + NBUF - 1, 2 * p, 2 * p + 2);
 #pragma unroll
             for (int q = 0; q + p < S; ++q) {
-                acc_re[p + q] = T::mfma32(bi, fa[q][1], acc_re[p + q]);
-                acc_im[p + q] = T::mfma32(bn, fa[q][0], acc_im[p + q]);
+                acc_re[A::idx(p, q)] = T::mfma32(bi, fa[q][1], acc_re[A::idx(p, q)]);
+                acc_im[A::idx(p, q)] = T::mfma32(bn, fa[q][0], acc_im[A::idx(p, q)]);
             }
         }
     }
@@ -479,9 +524,10 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
     OzTile ct;
     (void)src.get(0, ct);
     int ck = 0, g = 0;
-    acc_t acc_re[S], acc_im[S];
+    typedef OzAcc<D, S> A;
+    acc_t acc_re[A::N], acc_im[A::N];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
+    for (int t = 0; t < A::N; ++t) {
         acc_re[t] = acc_t{};
         acc_im[t] = acc_t{};
     }
@@ -491,8 +537,13 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
         sm = 0.0;
 #pragma unroll
         for (int t = S - 1; t >= 0; --t) {
-            sr = sr * 0x1p-7 + (double)acc_re[t][r];
-            sm = sm * 0x1p-7 + (double)acc_im[t][r];
+            double vr = (double)acc_re[A::first(t)][r], vm = (double)acc_im[A::first(t)][r];
+            if (A::SPLIT && t == 3) {  // the level's second half (exact sum: both below 2^24)
+                vr += (double)acc_re[A::SPLIT ? 4 : 0][r];
+                vm += (double)acc_im[A::SPLIT ? 4 : 0][r];
+            }
+            sr = sr * A::INV_BASE + vr;
+            sm = sm * A::INV_BASE + vm;
         }
     };
     // first: the tile's first chunk (store); last: its last (scale).  Clears
@@ -528,7 +579,7 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
             }
         }
 #pragma unroll
-        for (int t = 0; t < S; ++t) {
+        for (int t = 0; t < A::N; ++t) {
             acc_re[t] = acc_t{};
             acc_im[t] = acc_t{};
         }
@@ -567,14 +618,14 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
                     const frag bn = T::neg(bi);
 #pragma unroll
                     for (int q = 0; q + p < S; ++q) {
-                        acc_re[p + q] = T::mfma32(br, fa[q][0], acc_re[p + q]);
-                        acc_im[p + q] = T::mfma32(br, fa[q][1], acc_im[p + q]);
+                        acc_re[A::idx(p, q)] = T::mfma32(br, fa[q][0], acc_re[A::idx(p, q)]);
+                        acc_im[A::idx(p, q)] = T::mfma32(br, fa[q][1], acc_im[A::idx(p, q)]);
                     }
                     if (prefetch) issue_units(2 * p, 2 * p + 2);
 #pragma unroll
                     for (int q = 0; q + p < S; ++q) {
-                        acc_re[p + q] = T::mfma32(bi, fa[q][1], acc_re[p + q]);
-                        acc_im[p + q] = T::mfma32(bn, fa[q][0], acc_im[p + q]);
+                        acc_re[A::idx(p, q)] = T::mfma32(bi, fa[q][1], acc_re[A::idx(p, q)]);
+                        acc_im[A::idx(p, q)] = T::mfma32(bn, fa[q][0], acc_im[A::idx(p, q)]);
                     }
                 }
                 if (prefetch) advance_issue();
@@ -598,7 +649,7 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
                 }
             }
 #pragma unroll
-            for (int t = 0; t < S; ++t) {
+            for (int t = 0; t < A::N; ++t) {
                 acc_re[t] = acc_t{};
                 acc_im[t] = acc_t{};
             }

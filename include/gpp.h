@@ -86,21 +86,27 @@ typedef enum {
      * followed by a fused FP64 pole-factor reduce. */
     GPP_FORM_FACTORED_F64 = 2,
     /* As GPP_FORM_FACTORED_F64 with the Gram product split into FP16 slices
-     * (Ozaki scheme: 7-bit digits under a power-of-two scale per column),
-     * multiplied on FP16 MFMA with exact FP32 accumulation and recombined in
-     * FP64; slices per operand from GPP_OPT_OZAKI_SLICES. */
+     * (Ozaki scheme: digits in base 254 -- base 128 at 6 slices -- under a
+     * power-of-two scale per column), multiplied on FP16 MFMA with FP32
+     * accumulation (exact for the leading digit levels; the rest round below
+     * the split's truncation) and recombined in FP64; slices per operand from
+     * GPP_OPT_OZAKI_SLICES. */
     GPP_FORM_FACTORED_OZAKI = 3,
-    /* As GPP_FORM_FACTORED_OZAKI with the same digits held in INT8 and
-     * multiplied on INT8 MFMA with exact INT32 accumulation (flushed to FP64
-     * every 32768 bands). */
+    /* As GPP_FORM_FACTORED_OZAKI with the digits (base 254 at every slice
+     * count) held in INT8 and multiplied on INT8 MFMA with exact INT32
+     * accumulation (flushed to FP64 every 8192 bands). */
     GPP_FORM_FACTORED_OZAKI_I8 = 4
 } gpp_form;
 
 /* Tunable options of a context (gpp_ctx_set_option / gpp_ctx_get_option). */
 typedef enum {
-    /* Slices per operand of GPP_FORM_FACTORED_OZAKI, 3..6 (default 6).  Each
-     * slice carries 7 bits; the split truncates at about 2^(-7 S) of the
-     * column scale. */
+    /* Slices per operand of the Ozaki forms, 3..6 (default 6).  The first
+     * slice carries 7 bits, each later one about 8 (base 254): the split
+     * truncates at about 2^-39 of the column scale at 5 slices and 2^-47 at
+     * 6 (FP16 at 6 slices: 7-bit digits, 2^-42).  5 slices run 15 digit
+     * products instead of 21 (about 25 % faster) for problems whose sums do
+     * not cancel much (2.4e-11 at 512/2/32768/1638; 3.1e-10 at
+     * 1536/8/65536/3276 with 16 frequencies, where 6 are needed). */
     GPP_OPT_OZAKI_SLICES = 1,
     /* Block layout of the Ozaki Gram kernel (all give bit-identical results):
      * 0 = 64x64 tiles, 2x2 waves of 32x32 (one wave per SIMD); 1 = 64x64
