@@ -139,6 +139,29 @@ def test_batch_ozaki_vs_oracle_and_single_path(ctx, oracle, form):
             np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{form} {k}")
 
 
+@pytest.mark.parametrize("form", ["factored-ozaki-i8", "factored-ozaki"])
+def test_batch_ozaki_five_slices(ctx, oracle, form):
+    """At 5 slices (base-254 digits; FP16 sums its level-3 products in two
+    accumulators) the batch's digit product equals the single-problem one
+    bitwise, and every instance is within 1e-10 of the oracle relative to
+    each output's modulus."""
+    import pygpp
+    probs = [pygpp.Problem(*s) for s in SHAPES]
+    seeds = [71 + i for i in range(len(probs))]
+    try:
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, 5)
+        with pygpp.Batch(ctx, probs, seeds=seeds) as bs:
+            outs = bs.sigma(form)
+        for p, s, out in zip(probs, seeds, outs):
+            ctx.generate_seeded(p, s)
+            ref = ctx.sigma(form)
+            for k in KEYS:
+                np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{form} {k}")
+            assert_close_modulus(out, oracle.sigma(pygpp.fill_seeded(p, s), nvband=p.nvband, threads=8), TOL, KEYS)
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_OZAKI_SLICES, 6)
+
+
 def test_batch_switches_forms(ctx):
     """One batch evaluated in turn with the FP64, INT8 and FP16 Gram products
     (each re-plans the workspace and recaptures the graph) and back."""
