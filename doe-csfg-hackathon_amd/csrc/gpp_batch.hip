@@ -1,16 +1,18 @@
 // gpp_batch.hip -- many independent GPP sigma problems of uneven size in one
 // pass of the factored FP64 form.
 //
-// Each stage of the factored form (gpp_factored_dev.hpp) becomes ONE
-// persistent launch over a work list that spans every instance:
+// Each stage of the factored form (gpp_factored_dev.hpp) becomes ONE launch
+// over a work list that spans every instance (the reduce: one per occupied
+// band class):
 //   achstemp  (instance, band) items, one wave each, also gathering Bg
 //   Gram      (instance, G' tile, G tile) items of the FP64-MFMA product
 //   reduce    (instance, column, G-vector split) items, every frequency
 //   final     one block per instance
-// Lists are built on the host, sorted by decreasing cost, and walked with a
-// static round-robin (block b takes items b, b+G, b+2G, ...), so every item is
-// computed by exactly one block (or wave) and results do not depend on
-// scheduling.  The launches are captured once into a hipGraph and replayed.
+// Lists are built on the host, sorted by decreasing cost; the persistent
+// launches walk them with a static round-robin (block b takes items b, b+G,
+// b+2G, ...), the reduce runs one block per item.  Every item is computed by
+// exactly one block (or wave), so results do not depend on scheduling.  The
+// launches are captured once into a hipGraph and replayed.
 //
 // The Ozaki kinds replace the Gram launch by three: column exponents and
 // digit planes of every instance's Bg and aqsn (gpp_ozaki_dev.hpp), then the
@@ -144,27 +146,27 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 }
 
 // Reduce items of instances whose occupied band count is NV (0: any count);
-// NEG: instances with a negative frequency (generic count).  Two waves per
-// SIMD above three frequencies: three spill 10-22 VGPRs in the batch classes
-// and time the same on the 64-instance batch.
+// NEG: instances with a negative frequency (generic count).  One block per
+// item (the class's items in decreasing cost, which the dispatcher hands out
+// in order): without a persistent item loop the kernels fit three waves per
+// SIMD, as the single-problem reduce does (a persistent loop took 169-186
+// VGPRs, two waves); the negative-frequency class keeps two above four
+// frequencies (it spilled 75 registers at three).
 template <int FC, int NV, bool NEG = false>
-__global__ __launch_bounds__(kRedThreads, FC <= 3 ? 3 : 2) void k_batch_reduce(const BInst *__restrict__ insts,
-                                                              const RedItem *__restrict__ items, int nitems,
-                                                              int chunk)
+__global__ __launch_bounds__(kRedThreads, NEG && FC > 4 ? 2 : 3) void k_batch_reduce(const BInst *__restrict__ insts,
+                                                                 const RedItem *__restrict__ items, int chunk)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const RedItem it = items[w];
-        const BInst &b = insts[it.inst];
-        ReduceArgs ra;
-        ra.mu = b.mu; ra.ieps = b.ieps; ra.wtilde = b.wtilde; ra.aqsn = b.aqsn; ra.bg = b.bg; ra.idx = b.idx;
-        ra.vcoul = b.vcoul; ra.wx = b.wx; ra.nv = b.nv; ra.ncouls = b.ncouls; ra.ngpown = b.ngpown;
-        ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
-        ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
-        ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1; ra.part_occ = nullptr;
-        ra.fpb = b.nfreq;
-        pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC), NEG>(ra, it.j, it.split, 0, b.nfreq, scratch);
-    }
+    const RedItem it = items[blockIdx.x];
+    const BInst &b = insts[it.inst];
+    ReduceArgs ra;
+    ra.mu = b.mu; ra.ieps = b.ieps; ra.wtilde = b.wtilde; ra.aqsn = b.aqsn; ra.bg = b.bg; ra.idx = b.idx;
+    ra.vcoul = b.vcoul; ra.wx = b.wx; ra.nv = b.nv; ra.ncouls = b.ncouls; ra.ngpown = b.ngpown;
+    ra.nfreq = b.nfreq; ra.nsplit = b.nsplit; ra.chunk = chunk; ra.occ = b.occ;
+    ra.col_ach = b.col_ach; ra.col_asx = b.col_asx;
+    ra.p2 = nullptr; ra.br1 = nullptr; ra.acht_iw = -1; ra.part_occ = nullptr;
+    ra.fpb = b.nfreq;
+    pole_reduce_item<FC, NV, false, (FC > 4 ? 4 : FC), NEG>(ra, it.j, it.split, 0, b.nfreq, scratch);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_batch_final(const BInst *__restrict__ insts)
@@ -391,20 +393,20 @@ void launch_batch_oz(int S, int G, hipStream_t s, const BInst *insts, const int4
 }
 
 template <int FC>
-void launch_batch_reduce(int cls, dim3 grid, hipStream_t s, const BInst *insts, const RedItem *ri, int nr, int chunk)
+void launch_batch_reduce(int cls, dim3 grid, hipStream_t s, const BInst *insts, const RedItem *ri, int chunk)
 {
     switch (cls) {
-    case 2: hipLaunchKernelGGL((k_batch_reduce<FC, 2>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 3: hipLaunchKernelGGL((k_batch_reduce<FC, 3>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 4: hipLaunchKernelGGL((k_batch_reduce<FC, 4>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 5: hipLaunchKernelGGL((k_batch_reduce<FC, 5>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 6: hipLaunchKernelGGL((k_batch_reduce<FC, 6>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 7: hipLaunchKernelGGL((k_batch_reduce<FC, 7>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
-    case 8: hipLaunchKernelGGL((k_batch_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    case 2: hipLaunchKernelGGL((k_batch_reduce<FC, 2>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 3: hipLaunchKernelGGL((k_batch_reduce<FC, 3>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 4: hipLaunchKernelGGL((k_batch_reduce<FC, 4>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 5: hipLaunchKernelGGL((k_batch_reduce<FC, 5>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 6: hipLaunchKernelGGL((k_batch_reduce<FC, 6>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 7: hipLaunchKernelGGL((k_batch_reduce<FC, 7>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
+    case 8: hipLaunchKernelGGL((k_batch_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
     case kRedClassNeg:
-        hipLaunchKernelGGL((k_batch_reduce<FC, 0, true>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk);
+        hipLaunchKernelGGL((k_batch_reduce<FC, 0, true>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk);
         break;
-    default: hipLaunchKernelGGL((k_batch_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, insts, ri, nr, chunk); break;
+    default: hipLaunchKernelGGL((k_batch_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
     }
 }
 
@@ -450,11 +452,11 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
             const int b0 = pl.red_class_start[c], nr = pl.red_class_start[c + 1] - b0;
             if (nr == 0) continue;
             const RedItem *ri = reinterpret_cast<const RedItem *>(w + pl.tab_red) + b0;
-            const dim3 grid(std::max(1, std::min(G, nr)));
+            const dim3 grid(nr);  // one block per item
             if (pl.fc == 4)
-                launch_batch_reduce<4>(c, grid, s, insts, ri, nr, pl.kind.red_chunk);
+                launch_batch_reduce<4>(c, grid, s, insts, ri, pl.kind.red_chunk);
             else
-                launch_batch_reduce<8>(c, grid, s, insts, ri, nr, pl.kind.red_chunk);
+                launch_batch_reduce<8>(c, grid, s, insts, ri, pl.kind.red_chunk);
         }
     }
     if (which & 8)
