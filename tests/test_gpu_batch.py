@@ -1,5 +1,6 @@
-"""The batched factored forms: many uneven problems through one persistent
-launch per stage, replayed from a captured hipGraph.  Each instance must
+"""The batched factored forms: many uneven problems through one launch per
+stage (the pole reduce: one per occupied band class), replayed from a
+captured hipGraph.  Each instance must
 match the oracle -- the FP64 Gram product within 1e-10 per real / imaginary
 component, the Ozaki INT8 / FP16 products within 1e-10 of each output's
 modulus (the digit split's error bound is normwise) -- and, for the same tile
