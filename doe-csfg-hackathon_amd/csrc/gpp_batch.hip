@@ -447,7 +447,7 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
         }
     }
     if (which & 4) {
-        // One persistent launch per occupied band class present.
+        // One launch per occupied band class present, one block per item.
         for (int c = 0; c < kRedClasses; ++c) {
             const int b0 = pl.red_class_start[c], nr = pl.red_class_start[c + 1] - b0;
             if (nr == 0) continue;
