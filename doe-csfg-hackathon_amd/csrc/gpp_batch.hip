@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 // VGPRs, two waves); the negative-frequency class keeps two above four
 // frequencies (it spilled 75 registers at three).
 template <int FC, int NV, bool NEG = false>
-__global__ __launch_bounds__(kRedThreads, NEG && FC > 4 ? 2 : 3) void k_batch_reduce(const BInst *__restrict__ insts,
+__global__ __launch_bounds__(kRedThreads, FC > 8 || (NEG && FC > 4) ? 2 : 3) void k_batch_reduce(const BInst *__restrict__ insts,
                                                                  const RedItem *__restrict__ items, int chunk)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
@@ -191,6 +191,20 @@ constexpr int gram_ti(int tile) { return tile == 0 || tile == 1 ? 128 : 64; }
 // class 9 the instances with a negative frequency (any count).
 constexpr int kRedClasses = 10;
 constexpr int kRedClassNeg = 9;
+// Frequencies per reduce pass of a batch (GPP_OPT_REDUCE_FREQS: 4, 16, else 8).
+inline int batch_fc(int red_fc) { return red_fc == 4 ? 4 : red_fc == 16 ? 16 : 8; }
+// Frequency slots one reduce item evaluates in pole_reduce_item's passes of
+// fc, idle slots of the last passes included.
+inline int reduce_slots(int nfreq, int fc)
+{
+    if (fc == 16) {
+        int slots = 0, r = nfreq;
+        for (; r > 12; r -= 16) slots += 16;
+        if (r > 4) slots += 8, r -= 8;
+        return r > 0 ? slots + 4 : slots;
+    }
+    return (nfreq / fc) * fc + (nfreq % fc == 0 ? 0 : nfreq % fc <= 4 ? 4 : fc);
+}
 inline int red_class(const DevProblem &p) { return p.wx_neg ? kRedClassNeg : p.nv >= 2 && p.nv <= 8 ? p.nv : 0; }
 
 BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, size_t *work_bytes)
@@ -247,11 +261,12 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
         const long K = p.nb - p.nv;
         for (int i0 = 0; i0 < p.ncouls; i0 += ti)
             for (int j0 = 0; j0 < p.ngpown; j0 += tj) g.push_back({K, GramItem{i, j0, i0, 0}});
-        // One FC for the whole launch: an item runs all frequencies of its
-        // (column, G range) in passes of FC, the last one 4 wide when at most
-        // 4 remain (masked slots beyond nfreq still cost).
-        const int fc = k.red_fc == 4 ? 4 : 8;
-        const int slots = (p.nfreq / fc) * fc + (p.nfreq % fc == 0 ? 0 : p.nfreq % fc <= 4 ? 4 : fc);
+        // One FC for the whole launch (at most 8 for the negative-frequency
+        // class): an item runs all frequencies of its (column, G range) in
+        // passes of FC with narrower last passes (masked slots beyond nfreq
+        // still cost).
+        const int fc = p.wx_neg ? std::min(batch_fc(k.red_fc), 8) : batch_fc(k.red_fc);
+        const int slots = reduce_slots(p.nfreq, fc);
         const int nsplit = pl.inst_off[i].nsplit;
         for (int j = 0; j < p.ngpown; ++j)
             for (int sp = 0; sp < nsplit; ++sp) {
@@ -300,7 +315,7 @@ BatchPlan plan_batch(const std::vector<DevProblem> &probs, const GramKind &k, si
     pl.tab_oz_split = off; off += align256(sizeof(int4) * (pl.oz_split_items.size() + 1));
     pl.tab_oz_gram = off;  off += align256(sizeof(GramItem) * (pl.oz_gram_items.size() + 1));
     pl.total = off;
-    pl.fc = k.red_fc == 4 ? 4 : 8;
+    pl.fc = batch_fc(k.red_fc);
     if (work_bytes) *work_bytes = pl.total;
     return pl;
 }
@@ -403,8 +418,8 @@ void launch_batch_reduce(int cls, dim3 grid, hipStream_t s, const BInst *insts, 
     case 6: hipLaunchKernelGGL((k_batch_reduce<FC, 6>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
     case 7: hipLaunchKernelGGL((k_batch_reduce<FC, 7>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
     case 8: hipLaunchKernelGGL((k_batch_reduce<FC, 8>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
-    case kRedClassNeg:
-        hipLaunchKernelGGL((k_batch_reduce<FC, 0, true>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk);
+    case kRedClassNeg:  // at most 8 per pass (the 16-wide variant spills)
+        hipLaunchKernelGGL((k_batch_reduce<(FC > 8 ? 8 : FC), 0, true>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk);
         break;
     default: hipLaunchKernelGGL((k_batch_reduce<FC, 0>), grid, dim3(kRedThreads), 0, s, insts, ri, chunk); break;
     }
@@ -455,6 +470,8 @@ hipError_t launch_batch(const BatchPlan &pl, void *work, int persistent_blocks, 
             const dim3 grid(nr);  // one block per item
             if (pl.fc == 4)
                 launch_batch_reduce<4>(c, grid, s, insts, ri, pl.kind.red_chunk);
+            else if (pl.fc == 16)
+                launch_batch_reduce<16>(c, grid, s, insts, ri, pl.kind.red_chunk);
             else
                 launch_batch_reduce<8>(c, grid, s, insts, ri, pl.kind.red_chunk);
         }

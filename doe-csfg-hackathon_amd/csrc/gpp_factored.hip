@@ -279,8 +279,8 @@ void launch_gram_cfg(const DevProblem &p, const cplx *bg, cplx *mu, hipStream_t 
 template <int FC, bool P2>
 void launch_reduce_p2(const ReduceArgs &a, dim3 grid, hipStream_t s)
 {
-    if (a.neg) {
-        hipLaunchKernelGGL((k_pole_reduce<FC, 0, P2, true>), grid, dim3(kRedThreads), 0, s, a);
+    if (a.neg) {  // at most 8 per pass (the 16-wide variant spills)
+        hipLaunchKernelGGL((k_pole_reduce<(FC > 8 ? 8 : FC), 0, P2, true>), grid, dim3(kRedThreads), 0, s, a);
         return;
     }
     switch (a.nv) {  // the occupied band counts of the common shapes
@@ -338,11 +338,12 @@ hipError_t launch_gram_f64(const DevProblem &p, int tile, const cplx *bg, cplx *
 }
 
 // red_fc 0: passes of 16 from 16 frequencies up, none negative (C3, 16
-// frequencies: one pass, 7.19 -> 6.74 ms against two of 8; the
-// negative-frequency kernel spills at that width), else 8.
+// frequencies: one pass, 7.19 -> 6.74 ms against two of 8), else 8.  With a
+// negative frequency at most 8 (that kernel spills at 16).
 hipError_t launch_pole_reduce(const ReduceArgs &ra, int red_fc, hipStream_t s)
 {
-    const int fc = red_fc == 4 || red_fc == 8 || red_fc == 16 ? red_fc : ra.nfreq >= 16 && !ra.neg ? 16 : 8;
+    int fc = red_fc == 4 || red_fc == 8 || red_fc == 16 ? red_fc : ra.nfreq >= 16 && !ra.neg ? 16 : 8;
+    if (ra.neg && fc > 8) fc = 8;
     switch (ra.nfreq < 4 ? ra.nfreq : fc) {
     case 1: launch_reduce_fc<1>(ra, s); break;
     case 2: launch_reduce_fc<2>(ra, s); break;

@@ -521,6 +521,16 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_off(), iw0);
     } else if (a.nfreq < 4) {  // at most 3 frequencies: one pass FCT (>= 4) wide
         if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_off(), iw0);
+    } else if constexpr (FC >= 16) {
+        // passes of 16 while more than 12 remain, then at most one of 8 and
+        // one of 4, so no pass runs more than 3 idle frequencies beyond the
+        // first (24: 16 + 8 instead of 16 + 16)
+        for (; iw_end - iw0 > 12; iw0 += 16) pass(std::integral_constant<int, 16>(), fold_on(), iw0);
+        if (iw_end - iw0 > 4) {
+            pass(std::integral_constant<int, 8>(), fold_on(), iw0);
+            iw0 += 8;
+        }
+        if (iw0 < iw_end) pass(std::integral_constant<int, 4>(), fold_on(), iw0);
     } else {
         for (; iw_end - iw0 > FCT; iw0 += FC) pass(std::integral_constant<int, FC>(), fold_on(), iw0);
         if (iw0 < iw_end) pass(std::integral_constant<int, FCT>(), fold_on(), iw0);

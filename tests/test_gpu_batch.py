@@ -199,6 +199,38 @@ def test_graph_replays_match_eager(ctx):
                     np.testing.assert_array_equal(g[k], e[k])
 
 
+def test_batch_reduce_freqs_bitwise(ctx, oracle):
+    """The batched reduce in passes of 4, 8 or 16 frequencies (16: passes of
+    16, then 8 and 4; the negative-frequency class at most 8) regroups the
+    same per-frequency sums: bitwise equal for every pass width, within 1e-10
+    of the oracle."""
+    import pygpp
+    arrays = make_arrays(seed0=500)
+    arrays.append((gpp_inputs.seeded_inputs(26, 900, 30, nfreq=28, seed=512, dw=2.0 / 27), 4))
+    arrays.append((gpp_inputs.seeded_inputs(22, 700, 25, nfreq=24, seed=513, dw=2.0 / 23), 2))
+    neg = gpp_inputs.seeded_inputs(10, 500, 25, nfreq=20, seed=514, wx_center=0.0, dw=0.5)
+    arrays.append((neg, 2))
+    base = None
+    try:
+        for fc in (4, 8, 16):
+            ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, fc)
+            b, dev = device_batch(ctx, arrays)
+            try:
+                outs = b.sigma()
+                assert b.last_timing()["reduce_freqs"] == fc
+            finally:
+                b.close()
+            if base is None:
+                base = outs
+                for (a, nv), out in zip(arrays, outs):
+                    assert_close(out, oracle.sigma(a, nvband=nv, threads=8), TOL, KEYS)
+            for i, (out, ref) in enumerate(zip(outs, base)):
+                for k in KEYS:
+                    np.testing.assert_array_equal(out[k], ref[k], err_msg=f"fc {fc} instance {i} {k}")
+    finally:
+        ctx.set_option(pygpp.GPP_OPT_REDUCE_FREQS, 0)
+
+
 def test_batch_autotune_keeps_accuracy(ctx, oracle):
     import pygpp
     arrays = make_arrays(seed0=300)

@@ -346,14 +346,18 @@ def test_reduce_freqs_option(ctx):
 ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 
-@pytest.mark.parametrize("nfreq", [4, 11, 13, 20])
+@pytest.mark.parametrize("nfreq", [4, 11, 13, 20, 28])
 def test_reduce_schedules_agree(ctx, oracle, nfreq):
-    """Frequencies per pass (4, 8 or 16, the last pass 4 wide when at most 4
-    remain) and per block (all, 4, 8) regroup the same per-frequency sums:
+    """Frequencies per pass (4, 8 or 16; with 8 the last pass is 4 wide when
+    at most 4 remain, with 16 passes of 16 run while more than 12 remain,
+    then one of 8 and one of 4: 28 = 16 + 8 + 4) and per block (all, 4, 8)
+    regroup the same per-frequency sums:
     bit-identical for one G-range size; every schedule, at G ranges of 256
     and 8192, within 1e-10 of the oracle."""
     import pygpp
-    arrs = gpp_inputs.seeded_inputs(30, 3000, 64, nfreq=nfreq, seed=70 + nfreq, dw=0.3)
+    # every frequency positive (negative ones run at most 8 per pass)
+    arrs = gpp_inputs.seeded_inputs(30, 3000, 64, nfreq=nfreq, seed=70 + nfreq, dw=0.3 if nfreq <= 20 else 0.25)
+    assert arrs["wx_array"].min() > 0
     ctx.upload(problem_of(arrs, 4), arrs)
     ref = oracle.sigma(arrs, nvband=4, threads=8)
     try:
