@@ -349,8 +349,10 @@ def main():
     # workspace, slower where that fits.
     # "...@5": the Ozaki form at 5 slices (39-bit digits) instead of the
     # default 6 (47 bits for INT8, 42 for FP16): 15 digit pairs instead of 21.
+    # "factored-ozaki-i8@5+acht": the fastest form with every output (its
+    # acht_n1_loc product runs on FP64 MFMA, as in factored-f64+acht).
     extra = ["factored-f64+acht"] + (["factored-f64-fused"] if NFREQ <= 3 else [])
-    extra += ["factored-ozaki-i8@5"] + (["factored-ozaki@5"] if args.workload == "c1" else [])
+    extra += ["factored-ozaki-i8@5"] + (["factored-ozaki@5", "factored-ozaki-i8@5+acht"] if args.workload == "c1" else [])
     for form in run_forms + extra:
         name, acht = (form[:-5], True) if form.endswith("+acht") else (form, None)
         fused = name.endswith("-fused")
