@@ -865,6 +865,10 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
         if (st + 1 < nst) load_piece(k_begin + (st + 1) * kAKS + h * kAK);
+        // as in the Gram product: MFMA section first, fragment reads
+        // interleaved with the MFMAs (C1 2.823 -> 2.799 ms, C3 33.96 -> 33.66)
+        __builtin_amdgcn_s_setprio(1);
+        __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
         for (int kk = 0; kk < kAK; kk += 4) {
             const int kr = (kk + lane / 16) ^ (lane & 7);  // every row read below is = lane (mod 8)
@@ -888,6 +892,7 @@ __device__ __forceinline__ void acht_f64_tile(const cplx *__restrict__ p2,
                 }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         if (st + 1 < nst) store_piece(buf ^ 1, h);
         }
         __syncthreads();
