@@ -207,9 +207,11 @@ __device__ __forceinline__ void occ_branch_tests(double wxf, cplx wt, double wdi
 #pragma clang fp contract(off)
     constexpr double limitone = 1.0 / (kTo1 * 4.0);
     constexpr double kEstLo1 = limitone * (1 - kBand), kEstHi1 = limitone * (1 + kBand);
-    constexpr double kEstLo2 = kTo1 * (1 - kBand), kEstHi2 = kTo1 * (1 + kBand);
+    constexpr double kEstHi2 = kTo1 * (1 + kBand);
     const double est = wt2 * rden;
-    const bool sure = (est < kEstLo2) | ((est > kEstHi2) & (est < kEstLo1)) | ((est > kEstHi1) & (est <= __DBL_MAX__));
+    // below the to1 band (|delw|^2 < 1e-6, where |wdiff| exceeds |wt| a
+    // thousandfold) is rare enough to take the exact path with the bands
+    const bool sure = ((est > kEstHi2) & (est < kEstLo1)) | ((est > kEstHi1) & (est <= __DBL_MAX__));
     bool below1 = est < kEstLo1;
     above2 = est > kEstHi2;
     if (!sure) {
