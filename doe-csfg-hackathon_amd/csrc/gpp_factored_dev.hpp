@@ -198,9 +198,12 @@ constexpr double kBand = 1.0 / 1099511627776.0;
 // est = wt2 * rden is within 1e-14 of the reference's rounded |delw|^2,
 // relative to it (each rounding of delw is bounded by |delw|, with or
 // without cancellation), so outside a 2^-40 band around the two thresholds
-// it decides as the reference does; inside it, and for a non-finite est, the
-// reference's delw and |delw|^2 are recomputed in its operation order with
-// the correctly rounded reciprocal (:30-32).
+// it decides as the reference does; inside it, below the lower one and for a
+// NaN est (|wdiff|^2 = 0 or beyond the double range: rcp_1nr returns NaN),
+// the reference's delw and |delw|^2 are recomputed in its operation order
+// with the correctly rounded reciprocal (:30-32).  An est of +inf (the
+// product overflowed) means a |delw|^2 of at least about 1.8e308, far above
+// both thresholds, where the reference decides alike (branch 2).
 __device__ __forceinline__ void occ_branch_tests(double wxf, cplx wt, double wdiffr, double rden, double wt2,
                                                  bool &branch1, bool &above2)
 {
@@ -211,7 +214,7 @@ __device__ __forceinline__ void occ_branch_tests(double wxf, cplx wt, double wdi
     const double est = wt2 * rden;
     // below the to1 band (|delw|^2 < 1e-6, where |wdiff| exceeds |wt| a
     // thousandfold) is rare enough to take the exact path with the bands
-    const bool sure = ((est > kEstHi2) & (est < kEstLo1)) | ((est > kEstHi1) & (est <= __DBL_MAX__));
+    const bool sure = ((est > kEstHi2) & (est < kEstLo1)) | (est > kEstHi1);
     bool below1 = est < kEstLo1;
     above2 = est > kEstHi2;
     if (!sure) {
