@@ -197,13 +197,15 @@ inline int batch_fc(int red_fc) { return red_fc == 4 ? 4 : red_fc == 16 ? 16 : 8
 // fc, idle slots of the last passes included.
 inline int reduce_slots(int nfreq, int fc)
 {
+    if (fc == 4 || nfreq < 4) return (nfreq + 3) / 4 * 4;
+    int slots = 0, r = nfreq;
     if (fc == 16) {
-        int slots = 0, r = nfreq;
         for (; r > 12; r -= 16) slots += 16;
         if (r > 4) slots += 8, r -= 8;
         return r > 0 ? slots + 4 : slots;
     }
-    return (nfreq / fc) * fc + (nfreq % fc == 0 ? 0 : nfreq % fc <= 4 ? 4 : fc);
+    for (; r > 8; r -= 8) slots += 8;
+    return slots + (r > 6 ? 8 : r > 4 ? 6 : r > 2 ? 4 : r > 0 ? 2 : 0);
 }
 inline int red_class(const DevProblem &p) { return p.wx_neg ? kRedClassNeg : p.nv >= 2 && p.nv <= 8 ? p.nv : 0; }
 

@@ -168,8 +168,10 @@ __global__ __launch_bounds__(256) void k_pole_cols(const cplx *col_ach, const cp
 // the time, and a fourth hides more of it despite 17 spilled registers
 // (3 frequencies x 2 occupied bands: 0.510 -> 0.486 ms at 512/2/32768/1638);
 // the p2-storing and negative-frequency variants spill 30-44 and keep three.
+// The negative-frequency kernels of 8 frequencies per pass run two (their
+// cutoff code spilled 84-186 registers at three).
 template <int FC, int NV, bool P2, bool NEG = false>
-__global__ __launch_bounds__(kRedThreads, (FC <= 3 && !P2 && !NEG) ? 4 : FC > 8 ? 2 : 3) void k_pole_reduce(ReduceArgs a)
+__global__ __launch_bounds__(kRedThreads, (FC <= 3 && !P2 && !NEG) ? 4 : FC > 8 || (NEG && FC > 4) ? 2 : 3) void k_pole_reduce(ReduceArgs a)
 {
     __shared__ cplx scratch[kRedThreads / kWave];
     const int iw_begin = blockIdx.z * a.fpb;

@@ -127,7 +127,8 @@ typedef enum {
     /* Frequencies per pole-reduce pass when nfreq > 4: 4, 8 or 16; 0 = 16
      * from 16 frequencies up when none is negative, else 8 (default).  The
      * last passes are narrower: 8 then 4 wide with 16 (24 frequencies run
-     * 16 + 8), 4 wide with 8.  Negative frequencies run at most 8.  The
+     * 16 + 8), one of 8, 6, 4 or 2 with 8 (13 run 8 + 6).  Negative
+     * frequencies run at most 8.  The
      * batched form runs 4 for 4, 16 for 16 and 8 otherwise (16-frequency
      * passes timed slower on its uneven work lists, 11.9 against 11.5 ms). */
     GPP_OPT_REDUCE_FREQS = 5,

@@ -346,12 +346,12 @@ def test_reduce_freqs_option(ctx):
 ALL_FORMS = ["direct", "factored-f64", "factored-ozaki", "factored-ozaki-i8"]
 
 
-@pytest.mark.parametrize("nfreq", [4, 11, 13, 20, 28])
+@pytest.mark.parametrize("nfreq", [4, 11, 13, 18, 20, 28])
 def test_reduce_schedules_agree(ctx, oracle, nfreq):
-    """Frequencies per pass (4, 8 or 16; with 8 the last pass is 4 wide when
-    at most 4 remain, with 16 passes of 16 run while more than 12 remain,
-    then one of 8 and one of 4: 28 = 16 + 8 + 4) and per block (all, 4, 8)
-    regroup the same per-frequency sums:
+    """Frequencies per pass (4; 8: passes of 8 while more than 8 remain,
+    then one of 8, 6, 4 or 2, 13 = 8 + 6, 18 = 8 + 8 + 2; 16: passes of 16
+    while more than 12 remain, then one of 8 and one of 4, 28 = 16 + 8 + 4)
+    and per block (all, 4, 8) regroup the same per-frequency sums:
     bit-identical for one G-range size; every schedule, at G ranges of 256
     and 8192, within 1e-10 of the oracle."""
     import pygpp
