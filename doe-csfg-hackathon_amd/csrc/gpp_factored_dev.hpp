@@ -1,6 +1,6 @@
 // gpp_factored_dev.hpp -- device building blocks of the factored form, shared
 // by the single-instance kernels (gpp_factored.hip, one grid per stage) and
-// the batched persistent kernels (gpp_batch.hip, one work list per stage).
+// the batched kernels (gpp_batch.hip, one work list per stage).
 //
 // Factorisation (wx_array is band-independent, gppComplex.cpp:323-329):
 //   Mu[j,ig] = sum_{n1 >= nv} conj(aqsm[n1,igp(j)]) * aqsn[n1,ig]
