@@ -151,7 +151,8 @@ __global__ __launch_bounds__(256, 1) void k_batch_gram_oz(const BInst *__restric
 // in order): without a persistent item loop the kernels fit three waves per
 // SIMD, as the single-problem reduce does (a persistent loop took 169-186
 // VGPRs, two waves); the negative-frequency class keeps two above four
-// frequencies (it spilled 75 registers at three).
+// frequencies (it spilled 75 registers at three), and passes of 16 (asked
+// for with GPP_OPT_REDUCE_FREQS) run two, as in the single-problem reduce.
 template <int FC, int NV, bool NEG = false>
 __global__ __launch_bounds__(kRedThreads, FC > 8 || (NEG && FC > 4) ? 2 : 3) void k_batch_reduce(const BInst *__restrict__ insts,
                                                                  const RedItem *__restrict__ items, int chunk)
