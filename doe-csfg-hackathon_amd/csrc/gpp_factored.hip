@@ -167,7 +167,8 @@ __global__ __launch_bounds__(256) void k_pole_cols(const cplx *col_ach, const cp
 // kernels of at most three frequencies: their waves wait on memory half of
 // the time, and a fourth hides more of it despite 17 spilled registers
 // (3 frequencies x 2 occupied bands: 0.510 -> 0.486 ms at 512/2/32768/1638);
-// the p2-storing and negative-frequency variants spill 30-44 and keep three.
+// the p2-storing and negative-frequency variants spill 30-72 and keep three
+// (the p2 one at four: 0.70 -> 1.28 ms at 512/2/32768/1638).
 // The negative-frequency kernels of 8 frequencies per pass run two (their
 // cutoff code spilled 84-186 registers at three).
 template <int FC, int NV, bool P2, bool NEG = false>
