@@ -428,7 +428,9 @@ __device__ __forceinline__ void gram_tile32(const D *__restrict__ bd, const D *_
             if (j < ngpown && ig < ncouls) {
                 double sr, sm;
                 fold(r, sr, sm);
-                mout[(size_t)j * ncouls + ig] = cplx{ldexp(sr, sc[r]), ldexp(sm, sc[r])};
+                // non-temporal: written once, read back once by the pole reduce
+                __builtin_nontemporal_store(ldexp(sr, sc[r]), &mout[(size_t)j * ncouls + ig].re);
+                __builtin_nontemporal_store(ldexp(sm, sc[r]), &mout[(size_t)j * ncouls + ig].im);
             }
         }
     }
@@ -645,7 +647,10 @@ __device__ __forceinline__ void gram_tiles32(const Src &src, unsigned char (*lds
                 if (j < ct.ngpown && ig < ct.ncouls) {
                     double sr, sm;
                     fold(r, sr, sm);
-                    ct.mout[(size_t)j * ct.ncouls + ig] = cplx{ldexp(sr, sc[r]), ldexp(sm, sc[r])};
+                    // non-temporal: written once, read back once by the pole
+                    // reduce (C1 INT8 step 2.64 -> 2.60 ms, 5 slices 2.20 -> 2.16)
+                    __builtin_nontemporal_store(ldexp(sr, sc[r]), &ct.mout[(size_t)j * ct.ncouls + ig].re);
+                    __builtin_nontemporal_store(ldexp(sm, sc[r]), &ct.mout[(size_t)j * ct.ncouls + ig].im);
                 }
             }
 #pragma unroll
