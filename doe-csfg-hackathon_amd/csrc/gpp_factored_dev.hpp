@@ -330,18 +330,26 @@ __device__ __forceinline__ void pole_reduce_item(const ReduceArgs &a, int j, int
         // The next element's wtilde, I_eps and Mu load while this one is
         // evaluated (two elements in flight per thread; a third did not
         // help).
+        // Kernels of at most three frequencies make one pass: their three
+        // streams are read once, without keeping them in the caches.
+        auto ld = [](const cplx *p) -> cplx {
+            if constexpr (FC <= 3)
+                return cplx{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+            else
+                return *p;
+        };
         cplx wt_n = czero(), ie_n = czero(), mu_n = czero();
         if (ig_begin + (int)threadIdx.x < ig_end) {
-            wt_n = a.wtilde[row + ig_begin + threadIdx.x];
-            ie_n = a.ieps[row + ig_begin + threadIdx.x];
-            mu_n = a.mu[row + ig_begin + threadIdx.x];
+            wt_n = ld(a.wtilde + row + ig_begin + threadIdx.x);
+            ie_n = ld(a.ieps + row + ig_begin + threadIdx.x);
+            mu_n = ld(a.mu + row + ig_begin + threadIdx.x);
         }
         for (int ig = ig_begin + threadIdx.x; ig < ig_end; ig += blockDim.x) {
             const cplx wt = wt_n, ie = ie_n, mu = mu_n;
             if (ig + (int)blockDim.x < ig_end) {
-                wt_n = a.wtilde[row + ig + blockDim.x];
-                ie_n = a.ieps[row + ig + blockDim.x];
-                mu_n = a.mu[row + ig + blockDim.x];
+                wt_n = ld(a.wtilde + row + ig + blockDim.x);
+                ie_n = ld(a.ieps + row + ig + blockDim.x);
+                mu_n = ld(a.mu + row + ig + blockDim.x);
             }
             // P = delw I_eps = q conj(wdiff) / |wdiff|^2 with q = wt I_eps and
             // conj(wdiff) = (wx - wt.re, wt.im) =: (wdr, wt.im).  FOLD (problems
