@@ -185,7 +185,10 @@ __device__ __forceinline__ void split_piece(const cplx *__restrict__ src, int ld
     for (int kk = 0; kk < KP; ++kk) {
         const int k = kq * KP + kk;
         cplx v = czero();
-        if (c < ncols && k < K) v = src[(size_t)(k_begin + k) * ld + c];
+        if (c < ncols && k < K) {  // read once here: kept out of the caches, which the digits use
+            const cplx *q = src + (size_t)(k_begin + k) * ld + c;
+            v = cplx{__builtin_nontemporal_load(&q->re), __builtin_nontemporal_load(&q->im)};
+        }
         double yr = ldexp(v.re, 6 - e), yi = ldexp(v.im, 6 - e);  // |y| < 64, exact
 #pragma unroll
         for (int p = 0; p < S; ++p) {
